@@ -1,0 +1,181 @@
+/*
+ * lsim.h -- C ABI of the MI355X (gfx950) implementation of the larnd-sim-jax LUT simulation
+ * chain: quenching -> drift -> segment-to-pixel -> LUT induced current -> pixel x tick
+ * waveforms -> self-trigger ADC, and the adjoint of that chain for 7 detector parameters.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a negative LSIM_E_* code; lsim_last_error()
+ *     returns a thread-local message for the last failure.  Nothing throws across the ABI.
+ *   - All array arguments are device pointers owned by the caller unless stated otherwise; the
+ *     context owns only the tables derived from the response LUT.
+ *   - `stream` is a hipStream_t passed as void*.  All work is stream ordered; no entry point
+ *     below lsim_forward_prepare synchronises the host, so lsim_forward / lsim_backward can be
+ *     captured into a hipGraph.
+ *   - Results are bitwise reproducible for fixed inputs (no float atomics anywhere).
+ *
+ * Reference interfaces replaced (larnd-sim-jax, paths relative to the repository root):
+ *   lsim_ctx_create / lsim_lut_set : load_detector_properties + load_lut
+ *                                    (src/larndsim/consts_jax.py:220-385, 387-449)
+ *   lsim_forward_prepare          : simulate_drift_new + the unique main-pixel set
+ *                                    (src/larndsim/sim_jax.py:374-453, 716-725)
+ *   lsim_forward_rows             : simulate_signals + get_adc_values + digitize + parse_output
+ *                                    (src/larndsim/sim_jax.py:141-286, 738-769, 619-647;
+ *                                     src/larndsim/fee_jax.py:55-71, 168-279)
+ *   lsim_forward                  : simulate_wfs + simulate_stochastic, i.e. the body of
+ *                                    LUTSimulation.predict (optimize/strategies.py:91-105)
+ *   lsim_fee_forward              : simulate_stochastic on caller waveforms (sim_jax.py:738-769)
+ *   lsim_backward                 : the reverse-mode derivative jax.value_and_grad takes through
+ *                                    LUTSimulation.predict (optimize/fit_params.py:731)
+ */
+#ifndef LSIM_H
+#define LSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSIM_OK 0
+#define LSIM_E_ARG (-1)       /* invalid argument / unsupported configuration */
+#define LSIM_E_HIP (-2)       /* a HIP runtime call failed */
+#define LSIM_E_STATE (-3)     /* call order violated (e.g. no LUT set) */
+#define LSIM_E_SPACE (-4)     /* workspace too small */
+
+#define LSIM_MAX_TPC 8
+#define LSIM_MAX_TEMPLATES 128
+#define LSIM_MAX_ADC 16
+
+/* status bits written to lsim_fwd_out.status (device int32) */
+#define LSIM_STATUS_ROWS_OVERFLOW 1   /* more pixel rows than max_rows: extra rows skipped */
+#define LSIM_STATUS_BAD_EVENT 2       /* eventID >= n_events: segment ignored */
+
+typedef struct lsim_ctx lsim_ctx;
+
+/* Detector geometry (consts_jax.py:357-383).  tpc_borders[t][axis][0/1] in cm; for the z
+ * axis entry 0 is the anode and entry 1 the cathode. */
+typedef struct {
+    int32_t n_tpc;
+    int32_t n_pixels_x, n_pixels_y;
+    double pixel_pitch;
+    double tpc_borders[LSIM_MAX_TPC][3][2];
+} lsim_geometry;
+
+/* Static simulation configuration (the non-differentiable Params fields that shape arrays). */
+typedef struct {
+    int32_t number_pix_neighbors;        /* n: (2n+1)^2 neighbour pixels */
+    int32_t nb_tran_diff_bins;           /* 1, 3 or 5 */
+    int32_t nb_sampling_bins_per_pixel;  /* 10 */
+    int32_t signal_length;               /* L: samples per window (150) */
+    int32_t n_ticks;                     /* array ticks incl. the garbage column (2001) */
+    int32_t max_adc_values;              /* H <= LSIM_MAX_ADC (10) */
+    int32_t adc_interval;                /* self-trigger integration ticks (18) */
+    int32_t n_templates;                 /* longitudinal-diffusion templates (100) */
+    int32_t long_diff_extent;            /* Gaussian half width in ticks (20) */
+    double long_diff_template[LSIM_MAX_TEMPLATES]; /* template sigmas in ticks, ascending */
+} lsim_config;
+
+/* Per-call parameters (Params fields read by the path).  recombination_mode: 1 BOX, 2 BIRKS,
+ * 3 ELLIPSOID.  vdrift_eff / dvdrift_deField are get_vdrift(eField) and its derivative,
+ * computed by the caller (the `vdrift` field itself is never read, as in the reference). */
+typedef struct {
+    double eField, Ab, kb, lifetime, vdrift, long_diff, tran_diff;
+    double lArDensity, alpha, beta, R_param, MeVToElectrons;
+    double vdrift_eff, dvdrift_deField;
+    double shift_x, shift_y, shift_z, size_margin, t_sampling;
+    double threshold, gain, v_pedestal, v_cm, v_ref, adc_counts;
+    int32_t recombination_mode;
+    int32_t pad_;
+} lsim_params;
+
+/* Track segments.  Element k of a column is at ptr[k * stride]: stride 1 is SoA, stride F reads
+ * the reference's (N, F) float32 row-major array in place (pointers = base + column index).
+ * z_start, z_end and dx are read only for ELLIPSOID recombination (may be NULL otherwise). */
+typedef struct {
+    const float *x, *y, *z, *dE, *dEdx, *event_id;
+    const float *z_start, *z_end, *dx;
+    int64_t stride;
+    int64_t n;
+} lsim_segments;
+
+/* Forward outputs, all caller-allocated device buffers sized for `max_rows` pixel rows. */
+typedef struct {
+    int64_t *n_rows;     /* [1]  number of valid pixel rows P (sorted unique main pixels) */
+    int64_t *pixel_ids;  /* [max_rows] pixel id of each row, ascending */
+    float *wfs;          /* [max_rows][n_ticks-1] waveforms (NULL: not stored) */
+    float *adc;          /* [max_rows][H] digitised ADC counts (noise off) */
+    int32_t *ticks;      /* [max_rows][H] hit ticks (n_ticks-3 = no hit) */
+    void *tape;          /* [max_rows * lsim_tape_bytes()] self-trigger decisions for the
+                            adjoint (NULL: not stored) */
+    int64_t *n_hits;     /* [1]  number of compacted hits (parse_output) */
+    float *hit_adc, *hit_x, *hit_y, *hit_z, *hit_tick, *hit_prob; /* [max_rows*H] */
+    int64_t *hit_event, *hit_pixel;                                /* [max_rows*H] */
+    int32_t *hit_index;  /* [max_rows*H] row*H + k of each compacted hit */
+    int32_t *status;     /* [1]  LSIM_STATUS_* bits */
+} lsim_fwd_out;
+
+const char *lsim_last_error(void);
+int lsim_version(void);
+size_t lsim_tape_bytes(void);
+
+int lsim_ctx_create(lsim_ctx **out, const lsim_geometry *geo, const lsim_config *cfg);
+int lsim_ctx_destroy(lsim_ctx *ctx);
+
+/* Build the template tables from the raw (nx, ny, nt) float32 LUT on the device. */
+int lsim_lut_set(lsim_ctx *ctx, const float *raw_lut_dev, int32_t nx, int32_t ny, int32_t nt, void *stream);
+
+/* Workspace bytes for n_seg segments of n_events events, and an upper bound on pixel rows. */
+int lsim_forward_query(lsim_ctx *ctx, int64_t n_seg, int32_t n_events, size_t *ws_bytes, int64_t *max_rows_bound);
+
+/* Phase 1: per-segment quantities, sort by main pixel, pixel-row enumeration.  Writes the row
+ * count to n_rows_dev (device int64).  The workspace keeps the state phase 2 and the backward
+ * read. */
+int lsim_forward_prepare(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg, int32_t n_events,
+                         void *ws, size_t ws_bytes, int64_t *n_rows_dev, int32_t *status_dev, void *stream);
+
+/* Phase 2: waveforms of every row, self-trigger, digitisation, hit compaction.  n_seg and
+ * n_events must match the prepare call that filled `ws`. */
+int lsim_forward_rows(lsim_ctx *ctx, const lsim_params *p, int64_t n_seg, int32_t n_events, void *ws,
+                      size_t ws_bytes, int64_t max_rows, const lsim_fwd_out *out, void *stream);
+
+/* Both phases in one stream-ordered call (graph capturable). */
+int lsim_forward(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg, int32_t n_events, void *ws,
+                 size_t ws_bytes, int64_t max_rows, const lsim_fwd_out *out, void *stream);
+
+/* Workspace bytes of lsim_fee_forward for n_rows rows. */
+int lsim_fee_query(lsim_ctx *ctx, int64_t n_rows, size_t *ws_bytes);
+
+/* Self-trigger + digitisation + compaction of caller waveforms wfs[n_rows][n_ticks-1] with pixel
+ * ids pixel_ids[n_rows] (rows with id < 0 produce no hits).  out->n_rows/pixel_ids/wfs are
+ * ignored; the adc/ticks/hit arrays must hold n_rows rows; out->tape is written if not NULL. */
+int lsim_fee_forward(lsim_ctx *ctx, const lsim_params *p, const float *wfs, const int64_t *pixel_ids, int64_t n_rows,
+                     void *ws, size_t ws_bytes, const lsim_fwd_out *out, void *stream);
+
+/* Copy the per-segment records of the last prepare call (in main-pixel order) for inspection:
+ * rec_out[n_seg * 80 bytes] = {int bx, by, k0, idx; float n_e, f, a, b, c, wx[5], wy[5], pad},
+ * keys_out[n_seg] = cell key (>= number of cells for dropped segments), order_out[n_seg] =
+ * input index of each record. */
+int lsim_segment_records(lsim_ctx *ctx, int64_t n_seg, int32_t n_events, void *ws, size_t ws_bytes, void *rec_out,
+                         uint32_t *keys_out, uint32_t *order_out, void *stream);
+
+/* Tuning: waves per pixel-row workgroup of the waveform kernel (1, 2, 4 or 8; default 4). */
+int lsim_set_row_waves(int waves);
+
+/* Backward workspace bytes for n_seg segments and max_rows rows. */
+int lsim_backward_query(lsim_ctx *ctx, int64_t n_seg, int64_t max_rows, size_t *bwd_ws_bytes);
+
+/* Gradient of sum(d_adc * adc) + sum(d_z * hit_z) with respect to
+ * (Ab, kb, eField, lifetime, vdrift, long_diff, tran_diff), written to d_params[7] (device
+ * float64).  d_adc and d_z are [max_rows][H] cotangents on the digitised ADC grid and on the
+ * hit z grid (d_z may be NULL).  Requires the workspace and the tape of the matching forward
+ * call.  Integer decisions (bins, pixels, ticks, template index, self-trigger crossings) are
+ * held fixed, as reverse-mode AD through the reference does. */
+int lsim_backward(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg, int32_t n_events, void *ws,
+                  size_t ws_bytes, int64_t max_rows, const lsim_fwd_out *fwd, const float *d_adc, const float *d_z,
+                  void *bwd_ws, size_t bwd_ws_bytes, double *d_params, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSIM_H */

@@ -1,0 +1,565 @@
+// C ABI of the MI355X LUT simulation chain (declarations and conventions in include/lsim.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "backward.h"
+
+using namespace lsim;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(LSIM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Bump allocator over a caller buffer (256-byte aligned sub-buffers).
+struct Carve {
+  char* base;
+  size_t off = 0;
+  explicit Carve(void* b) : base((char*)b) {}
+  template <typename T>
+  T* take(int64_t n) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = base ? (T*)(base + off) : nullptr;
+    off += (size_t)(n > 0 ? n : 1) * sizeof(T);
+    return p;
+  }
+};
+
+}  // namespace
+
+struct lsim_ctx {
+  lsim_geometry geo;
+  lsim_config cfg;
+  int halo = 1;
+  int cm = 5;
+  // LUT tables
+  bool has_lut = false;
+  int lx = 0, ly = 0, nt = 0;
+  float* TM = nullptr;
+  float* TN = nullptr;
+  float2* DM = nullptr;
+  float2* DN = nullptr;
+  double* CTM = nullptr;
+  double* CTN = nullptr;
+};
+
+namespace {
+
+int make_dev(const lsim_ctx* c, const lsim_params* p, int n_events, Dev& d) {
+  std::memset(&d, 0, sizeof(d));
+  const lsim_geometry& g = c->geo;
+  const lsim_config& k = c->cfg;
+  d.ntpc = g.n_tpc;
+  d.nx = g.n_pixels_x;
+  d.ny = g.n_pixels_y;
+  d.nsb = k.nb_sampling_bins_per_pixel;
+  d.nb = k.nb_tran_diff_bins;
+  d.nbsym = (d.nb - 1) / 2;
+  d.nn = k.number_pix_neighbors;
+  d.halo = c->halo;
+  d.gx = d.nx + 2 * d.halo;
+  d.gy = d.ny + 2 * d.halo;
+  d.n_events = n_events;
+  d.n_cells = (int64_t)n_events * d.ntpc * d.gx * d.gy;
+  d.pitch = g.pixel_pitch;
+  d.w = g.pixel_pitch / d.nsb;
+  std::memcpy(d.tpc, g.tpc_borders, sizeof(d.tpc));
+  d.L = k.signal_length;
+  d.nt = c->nt;
+  d.nticks = k.n_ticks;
+  d.nout = k.n_ticks - 1;
+  d.lx = c->lx;
+  d.ly = c->ly;
+  d.cm = c->cm;
+  d.nm = k.n_templates;
+  d.ext = k.long_diff_extent;
+  d.H = k.max_adc_values;
+  d.interval = k.adc_interval;
+  for (int m = 0; m < LSIM_MAX_TEMPLATES; ++m) d.sig[m] = k.long_diff_template[m];
+  if (p) {
+    d.eField = p->eField;
+    d.Ab = p->Ab;
+    d.kb = p->kb;
+    d.lifetime = p->lifetime;
+    d.long_diff = p->long_diff;
+    d.tran_diff = p->tran_diff;
+    d.rho = p->lArDensity;
+    d.alpha = p->alpha;
+    d.beta = p->beta;
+    d.R_param = p->R_param;
+    d.mev = p->MeVToElectrons;
+    d.v = p->vdrift_eff;
+    d.dv = p->dvdrift_deField;
+    d.shift[0] = p->shift_x;
+    d.shift[1] = p->shift_y;
+    d.shift[2] = p->shift_z;
+    d.margin = p->size_margin;
+    d.ts = p->t_sampling;
+    d.thr = p->threshold;
+    d.gain = p->gain;
+    d.vped = p->v_pedestal;
+    d.vcm = p->v_cm;
+    d.vref = p->v_ref;
+    d.adcc = p->adc_counts;
+    d.recomb = p->recombination_mode;
+    if (d.recomb < 1 || d.recomb > 3) return fail(LSIM_E_ARG, "recombination_mode must be 1 (BOX), 2 (BIRKS) or 3 (ELLIPSOID)");
+    if (!(d.v > 0)) return fail(LSIM_E_ARG, "vdrift_eff must be > 0");
+    if (!(d.ts > 0)) return fail(LSIM_E_ARG, "t_sampling must be > 0");
+    // segments inside the margin may sit up to floor(margin / pitch) + 1 pixels off the plane
+    const int need = 1 + (int)std::floor(p->size_margin / g.pixel_pitch);
+    if (need > c->halo) return fail(LSIM_E_ARG, "size_margin too large for the cell halo");
+  }
+  if (n_events < 0) return fail(LSIM_E_ARG, "n_events must be >= 0");
+  if (d.n_cells >= (int64_t)0x7fffffff) return fail(LSIM_E_ARG, "too many events for 32-bit cell keys");
+  return LSIM_OK;
+}
+
+SegIn make_segin(const lsim_segments* s) {
+  SegIn in;
+  in.x = s->x;
+  in.y = s->y;
+  in.z = s->z;
+  in.dE = s->dE;
+  in.dEdx = s->dEdx;
+  in.ev = s->event_id;
+  in.zs = s->z_start;
+  in.ze = s->z_end;
+  in.dx = s->dx;
+  in.stride = s->stride;
+  in.n = s->n;
+  return in;
+}
+
+int check_segments(const lsim_segments* s, int recomb) {
+  if (!s) return fail(LSIM_E_ARG, "segments is NULL");
+  if (s->n < 0 || s->n >= (int64_t)0x7fffffff) return fail(LSIM_E_ARG, "segment count out of range");
+  if (s->n > 0 && (!s->x || !s->y || !s->z || !s->dE || !s->dEdx || !s->event_id))
+    return fail(LSIM_E_ARG, "segment column pointer is NULL");
+  if (recomb == 3 && s->n > 0 && (!s->z_start || !s->z_end || !s->dx))
+    return fail(LSIM_E_ARG, "ELLIPSOID recombination needs z_start, z_end and dx");
+  if (s->stride < 1) return fail(LSIM_E_ARG, "segment stride must be >= 1");
+  return LSIM_OK;
+}
+
+// Workspace layout shared by the forward phases and the backward.
+struct Ws {
+  uint32_t *k0, *v0, *k1, *v1;  // sort ping-pong
+  void* sort_scratch;
+  SegRec* rec;
+  int2* rng;
+  int* cell_row;
+  int* flag;
+  int* ridx;
+  int* row_cell;
+  int* n_rows;
+  int* hit_cnt;
+  int* hit_off;
+  int* scan_tmp;
+  int* n_hits32;
+  int* sorted_sel;  // which ping-pong buffer holds the sorted keys (host written flag is not needed)
+  size_t bytes;
+};
+
+Ws layout(void* base, int64_t n, int64_t n_cells) {
+  Carve c(base);
+  Ws w;
+  w.k0 = c.take<uint32_t>(n);
+  w.v0 = c.take<uint32_t>(n);
+  w.k1 = c.take<uint32_t>(n);
+  w.v1 = c.take<uint32_t>(n);
+  w.sort_scratch = c.take<char>((int64_t)radix_sort_scratch(n));
+  w.rec = c.take<SegRec>(n);
+  w.rng = c.take<int2>(n_cells);
+  w.cell_row = c.take<int>(n_cells);
+  w.flag = c.take<int>(n);
+  w.ridx = c.take<int>(n);
+  w.row_cell = c.take<int>(n);
+  w.n_rows = c.take<int>(4);
+  w.hit_cnt = c.take<int>(n);
+  w.hit_off = c.take<int>(n);
+  w.scan_tmp = c.take<int>(scan_tiles(n) + 4);
+  w.n_hits32 = c.take<int>(4);
+  w.sorted_sel = c.take<int>(4);
+  w.bytes = c.off + 256;
+  return w;
+}
+
+int bits_for(int64_t v) {
+  int b = 0;
+  while (b < 32 && ((int64_t)1 << b) <= v) ++b;
+  return b;
+}
+
+// The sort result lands in (k0, v0) or (k1, v1) depending on the pass count; the pass count is a
+// pure function of n_cells, so both phases agree without storing anything.
+void sorted_buffers(const Ws& w, int64_t n_cells, uint32_t*& keys, uint32_t*& vals) {
+  const int bits = bits_for(n_cells);
+  const int passes = (bits + 7) / 8;
+  keys = (passes & 1) ? w.k1 : w.k0;
+  vals = (passes & 1) ? w.v1 : w.v0;
+}
+
+template <int WAVES>
+void launch_rows(const Dev& d, const Tables& tb, const Ws& w, int64_t max_rows, const RowOut& ro, hipStream_t st) {
+  hipLaunchKernelGGL((k_rows_fwd<WAVES>), dim3((unsigned)max_rows), dim3(WAVES * kWave), 0, st, d, tb, w.rec, w.rng,
+                     w.row_cell, w.n_rows, max_rows, ro);
+}
+
+int g_row_waves = 4;
+
+void free_tables(lsim_ctx* c) {
+  (void)hipFree(c->TM);
+  (void)hipFree(c->TN);
+  (void)hipFree(c->DM);
+  (void)hipFree(c->DN);
+  (void)hipFree(c->CTM);
+  (void)hipFree(c->CTN);
+  c->TM = c->TN = nullptr;
+  c->DM = c->DN = nullptr;
+  c->CTM = c->CTN = nullptr;
+  c->has_lut = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lsim_last_error(void) { return g_err.c_str(); }
+
+int lsim_version(void) { return 1; }
+
+size_t lsim_tape_bytes(void) { return sizeof(FeeTape); }
+
+int lsim_set_row_waves(int waves) {
+  if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return fail(LSIM_E_ARG, "waves must be 1, 2, 4 or 8");
+  g_row_waves = waves;
+  return LSIM_OK;
+}
+
+int lsim_ctx_create(lsim_ctx** out, const lsim_geometry* geo, const lsim_config* cfg) {
+  if (!out || !geo || !cfg) return fail(LSIM_E_ARG, "NULL argument");
+  *out = nullptr;
+  if (geo->n_tpc < 1 || geo->n_tpc > LSIM_MAX_TPC) return fail(LSIM_E_ARG, "n_tpc out of range");
+  if (geo->n_pixels_x < 1 || geo->n_pixels_y < 1) return fail(LSIM_E_ARG, "pixel counts must be >= 1");
+  if (!(geo->pixel_pitch > 0)) return fail(LSIM_E_ARG, "pixel_pitch must be > 0");
+  const int nb = cfg->nb_tran_diff_bins, nsb = cfg->nb_sampling_bins_per_pixel;
+  if (nb != 1 && nb != 3 && nb != 5) return fail(LSIM_E_ARG, "nb_tran_diff_bins must be 1, 3 or 5");
+  if (nsb < 2 || (nsb & 1)) return fail(LSIM_E_ARG, "nb_sampling_bins_per_pixel must be even and >= 2");
+  if ((nb - 1) / 2 >= nsb) return fail(LSIM_E_ARG, "diffusion bins wider than a pixel are not supported");
+  if (cfg->number_pix_neighbors < 0 || cfg->number_pix_neighbors > 8) return fail(LSIM_E_ARG, "number_pix_neighbors out of range");
+  if (cfg->signal_length < 1 || cfg->signal_length + 3 > kStage) return fail(LSIM_E_ARG, "signal_length out of range");
+  if (cfg->n_ticks < 4 || cfg->n_ticks - 1 > kMaxOut) return fail(LSIM_E_ARG, "n_ticks out of range (<= 2049)");
+  if (cfg->max_adc_values < 1 || cfg->max_adc_values > kMaxH) return fail(LSIM_E_ARG, "max_adc_values out of range");
+  if (cfg->n_templates < 3 || cfg->n_templates > LSIM_MAX_TEMPLATES) return fail(LSIM_E_ARG, "n_templates out of range");
+  if (cfg->long_diff_extent < 0 || 2 * cfg->long_diff_extent + 1 > 128) return fail(LSIM_E_ARG, "long_diff_extent out of range");
+  for (int m = 0; m < cfg->n_templates; ++m)
+    if (!(cfg->long_diff_template[m] > 0) || (m && !(cfg->long_diff_template[m] > cfg->long_diff_template[m - 1])))
+      return fail(LSIM_E_ARG, "long_diff_template must be positive and strictly increasing");
+  lsim_ctx* c = new lsim_ctx();
+  c->geo = *geo;
+  c->cfg = *cfg;
+  c->halo = 2;
+  c->cm = nsb / 2;
+  *out = c;
+  return LSIM_OK;
+}
+
+int lsim_ctx_destroy(lsim_ctx* c) {
+  if (!c) return LSIM_OK;
+  free_tables(c);
+  delete c;
+  return LSIM_OK;
+}
+
+int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t nt, void* stream) {
+  if (!c || !raw) return fail(LSIM_E_ARG, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  const lsim_config& k = c->cfg;
+  if (nt < k.signal_length + 1 || nt > kMaxOut) return fail(LSIM_E_ARG, "LUT time axis out of range");
+  const int nsb = k.nb_sampling_bins_per_pixel, n = k.number_pix_neighbors;
+  const int maxc = nsb * n + nsb / 2 - 1;  // largest |offset| index a neighbour reads
+  if (nx <= maxc || ny <= maxc || nx < c->cm || ny < c->cm)
+    return fail(LSIM_E_ARG, "LUT too small for number_pix_neighbors (needs > " + std::to_string(maxc) + " bins per axis)");
+  if (c->has_lut && (nx != c->lx || ny != c->ly || nt != c->nt)) free_tables(c);
+  c->lx = nx;
+  c->ly = ny;
+  c->nt = nt;
+  const int64_t ms = (int64_t)k.n_templates * c->cm * c->cm, ns = (int64_t)nx * ny;
+  const int L = k.signal_length;
+  if (!c->has_lut) {
+    HIP_TRY(hipMalloc(&c->TM, ms * L * sizeof(float)));
+    HIP_TRY(hipMalloc(&c->TN, ns * L * sizeof(float)));
+    HIP_TRY(hipMalloc(&c->DM, ms * nt * sizeof(float2)));
+    HIP_TRY(hipMalloc(&c->DN, ns * nt * sizeof(float2)));
+    HIP_TRY(hipMalloc(&c->CTM, ms * (L + 1) * sizeof(double)));
+    HIP_TRY(hipMalloc(&c->CTN, ns * (L + 1) * sizeof(double)));
+  }
+  Dev d;
+  int rc = make_dev(c, nullptr, 1, d);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ms), dim3(256), 0, st, d, raw, 1, c->TM, c->DM, c->CTM);
+  hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ns), dim3(256), 0, st, d, raw, 0, c->TN, c->DN, c->CTN);
+  HIP_TRY(hipGetLastError());
+  c->has_lut = true;
+  return LSIM_OK;
+}
+
+int lsim_forward_query(lsim_ctx* c, int64_t n_seg, int32_t n_events, size_t* ws_bytes, int64_t* max_rows_bound) {
+  if (!c || !ws_bytes) return fail(LSIM_E_ARG, "NULL argument");
+  Dev d;
+  int rc = make_dev(c, nullptr, n_events, d);
+  if (rc) return rc;
+  Ws w = layout(nullptr, n_seg, d.n_cells);
+  *ws_bytes = w.bytes;
+  if (max_rows_bound) {
+    const int64_t plane_pixels = (int64_t)n_events * d.ntpc * d.nx * d.ny;
+    *max_rows_bound = n_seg < plane_pixels ? n_seg : plane_pixels;
+  }
+  return LSIM_OK;
+}
+
+int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws,
+                         size_t ws_bytes, int64_t* n_rows_dev, int32_t* status_dev, void* stream) {
+  if (!c || !p || !ws) return fail(LSIM_E_ARG, "NULL argument");
+  if (!c->has_lut) return fail(LSIM_E_STATE, "lsim_lut_set has not been called");
+  hipStream_t st = (hipStream_t)stream;
+  Dev d;
+  int rc = make_dev(c, p, n_events, d);
+  if (rc) return rc;
+  rc = check_segments(seg, d.recomb);
+  if (rc) return rc;
+  const int64_t n = seg->n;
+  Ws w = layout(ws, n, d.n_cells);
+  if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small: need " + std::to_string(w.bytes) + " bytes");
+  SegIn in = make_segin(seg);
+  int* status = status_dev;
+
+  fill<int2>(w.rng, d.n_cells, make_int2(0, 0), st);
+  fill<int>(w.cell_row, d.n_cells, -1, st);
+  fill<int>(w.n_rows, 1, 0, st);
+  if (n > 0) {
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_keys, dim3(g), dim3(256), 0, st, d, in, w.k0, w.v0, status);
+    hipError_t err;
+    const int bits = bits_for(d.n_cells);
+    radix_sort_pairs(w.k0, w.v0, w.k1, w.v1, n, bits, w.sort_scratch, st, &err);
+    HIP_TRY(err);
+    uint32_t *keys, *vals;
+    sorted_buffers(w, d.n_cells, keys, vals);
+    hipLaunchKernelGGL(k_records, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, w.rec);
+    hipLaunchKernelGGL(k_cells, dim3(g), dim3(256), 0, st, d, keys, n, w.rng, w.flag);
+    HIP_TRY((exclusive_scan<int, int>(w.flag, w.ridx, n, w.scan_tmp, w.n_rows, st)));
+    hipLaunchKernelGGL(k_rows_map, dim3(g), dim3(256), 0, st, d, keys, w.flag, w.ridx, n, w.row_cell, w.cell_row);
+  }
+  hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, w.n_rows, n_rows_dev, (int64_t)-1, (int*)nullptr);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t max_rows, const lsim_fwd_out* out,
+                             hipStream_t st) {
+  Tables tb{c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
+  RowOut ro;
+  ro.wfs = out->wfs;
+  ro.adc = out->adc;
+  ro.ticks = out->ticks;
+  ro.tape = (FeeTape*)out->tape;
+  ro.hit_cnt = w.hit_cnt;
+  ro.pixel_ids = out->pixel_ids;
+  hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, w.n_rows, out->n_rows, max_rows, out->status);
+  if (max_rows > 0) {
+    switch (g_row_waves) {
+      case 1: launch_rows<1>(d, tb, w, max_rows, ro, st); break;
+      case 2: launch_rows<2>(d, tb, w, max_rows, ro, st); break;
+      case 8: launch_rows<8>(d, tb, w, max_rows, ro, st); break;
+      default: launch_rows<4>(d, tb, w, max_rows, ro, st); break;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY((exclusive_scan<int, int>(w.hit_cnt, w.hit_off, max_rows, w.scan_tmp, w.n_hits32, st)));
+    if (out->hit_adc) {
+      HitOut h{out->hit_adc, out->hit_x, out->hit_y, out->hit_z, out->hit_tick, out->hit_prob, out->hit_event,
+               out->hit_pixel, out->hit_index};
+      const int64_t tot = max_rows * d.H;
+      hipLaunchKernelGGL(k_hits, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, d, out->pixel_ids, out->adc,
+                         out->ticks, w.hit_off, out->n_rows, max_rows, (int64_t)0, h);
+    }
+  } else {
+    fill<int>(w.n_hits32, 1, 0, st);
+  }
+  if (out->n_hits) hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, w.n_hits32, out->n_hits, (int64_t)-1, (int*)nullptr);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+static int check_out(const lsim_fwd_out* out) {
+  if (!out) return fail(LSIM_E_ARG, "out is NULL");
+  if (!out->n_rows || !out->pixel_ids || !out->adc || !out->ticks || !out->status)
+    return fail(LSIM_E_ARG, "out: n_rows, pixel_ids, adc, ticks and status are required");
+  if (out->hit_adc && (!out->hit_x || !out->hit_y || !out->hit_z || !out->hit_tick || !out->hit_prob || !out->hit_event ||
+                       !out->hit_pixel || !out->hit_index))
+    return fail(LSIM_E_ARG, "out: hit arrays must be all set or all NULL");
+  return LSIM_OK;
+}
+
+int lsim_forward_rows(lsim_ctx* c, const lsim_params* p, int64_t n_seg, int32_t n_events, void* ws, size_t ws_bytes,
+                      int64_t max_rows, const lsim_fwd_out* out, void* stream) {
+  if (!c || !p || !ws) return fail(LSIM_E_ARG, "NULL argument");
+  if (!c->has_lut) return fail(LSIM_E_STATE, "lsim_lut_set has not been called");
+  int rc = check_out(out);
+  if (rc) return rc;
+  Dev d;
+  rc = make_dev(c, p, n_events, d);
+  if (rc) return rc;
+  Ws w = layout(ws, n_seg, d.n_cells);
+  if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
+  // rows never exceed segments; the per-row workspace buffers hold max(n, 1) entries
+  if (max_rows > (n_seg > 1 ? n_seg : 1)) max_rows = n_seg > 1 ? n_seg : 1;
+  return forward_rows_impl(c, d, w, max_rows, out, (hipStream_t)stream);
+}
+
+int lsim_forward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws, size_t ws_bytes,
+                 int64_t max_rows, const lsim_fwd_out* out, void* stream) {
+  int rc = check_out(out);
+  if (rc) return rc;
+  if (!seg) return fail(LSIM_E_ARG, "segments is NULL");
+  fill<int>(out->status, 1, 0, (hipStream_t)stream);
+  rc = lsim_forward_prepare(c, p, seg, n_events, ws, ws_bytes, nullptr, out->status, stream);
+  if (rc) return rc;
+  return lsim_forward_rows(c, p, seg->n, n_events, ws, ws_bytes, max_rows, out, stream);
+}
+
+int lsim_fee_query(lsim_ctx* c, int64_t n_rows, size_t* bytes) {
+  if (!c || !bytes) return fail(LSIM_E_ARG, "NULL argument");
+  Carve cv(nullptr);
+  cv.take<int>(n_rows);
+  cv.take<int>(n_rows);
+  cv.take<int>(scan_tiles(n_rows) + 4);
+  cv.take<int>(4);
+  *bytes = cv.off + 256;
+  return LSIM_OK;
+}
+
+int lsim_fee_forward(lsim_ctx* c, const lsim_params* p, const float* wfs, const int64_t* pixel_ids, int64_t n_rows,
+                     void* ws, size_t ws_bytes, const lsim_fwd_out* out, void* stream) {
+  if (!c || !p || !out || !out->adc || !out->ticks || !ws) return fail(LSIM_E_ARG, "NULL argument");
+  if (n_rows < 0) return fail(LSIM_E_ARG, "n_rows < 0");
+  if (n_rows > 0 && (!wfs || !pixel_ids)) return fail(LSIM_E_ARG, "wfs / pixel_ids is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  Dev d;
+  int rc = make_dev(c, p, 1, d);
+  if (rc) return rc;
+  size_t need;
+  lsim_fee_query(c, n_rows, &need);
+  if (need > ws_bytes) return fail(LSIM_E_SPACE, "fee workspace too small");
+  Carve cv(ws);
+  int* cnt = cv.take<int>(n_rows);
+  int* off = cv.take<int>(n_rows);
+  int* tmp = cv.take<int>(scan_tiles(n_rows) + 4);
+  int* tot = cv.take<int>(4);
+  RowOut ro;
+  ro.wfs = nullptr;
+  ro.adc = out->adc;
+  ro.ticks = out->ticks;
+  ro.tape = (FeeTape*)out->tape;
+  ro.hit_cnt = cnt;
+  ro.pixel_ids = nullptr;
+  if (n_rows > 0) {
+    hipLaunchKernelGGL((k_fee_rows<256>), dim3((unsigned)n_rows), dim3(256), 0, st, d, wfs, pixel_ids, n_rows, ro);
+    HIP_TRY((exclusive_scan<int, int>(cnt, off, n_rows, tmp, tot, st)));
+    if (out->hit_adc) {
+      HitOut h{out->hit_adc, out->hit_x, out->hit_y, out->hit_z, out->hit_tick, out->hit_prob, out->hit_event,
+               out->hit_pixel, out->hit_index};
+      const int64_t t = n_rows * d.H;
+      hipLaunchKernelGGL(k_hits, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, st, d, pixel_ids, out->adc, out->ticks, off,
+                         (const int64_t*)nullptr, n_rows, n_rows, h);
+    }
+  } else {
+    fill<int>(tot, 1, 0, st);
+  }
+  if (out->n_hits) hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, tot, out->n_hits, (int64_t)-1, (int*)nullptr);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+int lsim_segment_records(lsim_ctx* c, int64_t n_seg, int32_t n_events, void* ws, size_t ws_bytes, void* rec_out,
+                         uint32_t* keys_out, uint32_t* order_out, void* stream) {
+  if (!c || !ws || !rec_out || !keys_out || !order_out) return fail(LSIM_E_ARG, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  Dev d;
+  int rc = make_dev(c, nullptr, n_events, d);
+  if (rc) return rc;
+  Ws w = layout(ws, n_seg, d.n_cells);
+  if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
+  if (n_seg <= 0) return LSIM_OK;
+  uint32_t *keys, *vals;
+  sorted_buffers(w, d.n_cells, keys, vals);
+  HIP_TRY(hipMemcpyAsync(rec_out, w.rec, (size_t)n_seg * sizeof(SegRec), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(keys_out, keys, (size_t)n_seg * 4, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(order_out, vals, (size_t)n_seg * 4, hipMemcpyDeviceToDevice, st));
+  return LSIM_OK;
+}
+
+int lsim_backward_query(lsim_ctx* c, int64_t n_seg, int64_t max_rows, size_t* bytes) {
+  if (!c || !bytes) return fail(LSIM_E_ARG, "NULL argument");
+  Carve cv(nullptr);
+  cv.take<Pieces>(max_rows);
+  cv.take<double>(max_rows);
+  cv.take<double>(((n_seg + 255) / 256) * kNGrad);
+  *bytes = cv.off + 256;
+  return LSIM_OK;
+}
+
+int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws, size_t ws_bytes,
+                  int64_t max_rows, const lsim_fwd_out* fwd, const float* d_adc, const float* d_z, void* bwd_ws,
+                  size_t bwd_ws_bytes, double* d_params, void* stream) {
+  if (!c || !p || !seg || !ws || !fwd || !fwd->tape || !d_adc || !bwd_ws || !d_params)
+    return fail(LSIM_E_ARG, "NULL argument (the forward tape is required)");
+  if (!c->has_lut) return fail(LSIM_E_STATE, "lsim_lut_set has not been called");
+  hipStream_t st = (hipStream_t)stream;
+  Dev d;
+  int rc = make_dev(c, p, n_events, d);
+  if (rc) return rc;
+  rc = check_segments(seg, d.recomb);
+  if (rc) return rc;
+  const int64_t n = seg->n;
+  Ws w = layout(ws, n, d.n_cells);
+  if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
+  if (max_rows > (n > 1 ? n : 1)) max_rows = n > 1 ? n : 1;
+  size_t need;
+  lsim_backward_query(c, n, max_rows, &need);
+  if (need > bwd_ws_bytes) return fail(LSIM_E_SPACE, "backward workspace too small");
+  Carve cv(bwd_ws);
+  Pieces* pieces = cv.take<Pieces>(max_rows);
+  double* dz = cv.take<double>(max_rows);
+  const int64_t nblk = (n + 255) / 256;
+  double* partial = cv.take<double>(nblk * kNGrad);
+  Tables tb{c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
+  SegIn in = make_segin(seg);
+  uint32_t *keys, *vals;
+  sorted_buffers(w, d.n_cells, keys, vals);
+  if (max_rows > 0)
+    hipLaunchKernelGGL(k_fee_adjoint, dim3((unsigned)((max_rows + 63) / 64)), dim3(64), 0, st, d, (const FeeTape*)fwd->tape,
+                       d_adc, d_z, fwd->n_rows, max_rows, pieces, dz);
+  if (nblk > 0)
+    hipLaunchKernelGGL(k_seg_adjoint, dim3((unsigned)nblk), dim3(256), 0, st, d, in, tb, keys, vals, n, w.cell_row, pieces,
+                       max_rows, partial);
+  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(256), 0, st, d, partial, nblk, dz, fwd->n_rows, max_rows, d_params);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+}  // extern "C"

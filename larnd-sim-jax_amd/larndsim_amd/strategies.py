@@ -1,0 +1,77 @@
+"""Simulation / loss strategies with the reference's plugin interface (optimize/strategies.py).
+
+``HipLUTSimulation`` is the drop-in for ``LUTSimulation`` (strategies.py:87-105): the same
+``predict(params, tracks, fields, rngkey) -> dict`` with keys adcs, pixel_x, pixel_y, pixel_z,
+ticks, hit_prob, event, hit_pixels, unique_pixels, wfs.  If any of the 7 physics parameters in
+``params`` is a torch tensor with ``requires_grad``, ``adcs`` and ``pixel_z`` carry autograd
+history through the hand-written adjoint.
+"""
+from __future__ import annotations
+
+import torch
+
+from .autograd import LUTSimFunction, needs_grad, theta_of
+from .fee import adc2charge
+from .sim import get_simulator, hits_dict, infer_n_events
+
+
+class SimulationStrategy:
+    def predict(self, params, tracks, fields, rngkey):
+        """Runs the simulation and returns a dictionary of outputs (strategies.py:80-85)."""
+        raise NotImplementedError
+
+
+class HipLUTSimulation(SimulationStrategy):
+    """LUT simulation on the GPU (strategies.py:87-105).  ``response`` is the raw (Nx, Ny, Nt) LUT
+    returned by ``consts.load_lut``.  Noise must be off (rngkey is accepted and unused)."""
+
+    def __init__(self, response):
+        self.response = response
+
+    def predict(self, params, tracks, fields, rngkey=None):
+        sim = get_simulator(params, self.response)
+        n_events = infer_n_events(tracks, fields)
+        if needs_grad(params):
+            theta = theta_of(params)
+            base = params.replace(**{n: float(v) for n, v in zip(
+                ("Ab", "kb", "eField", "lifetime", "vdrift", "long_diff", "tran_diff"), theta.detach().cpu().tolist())})
+            adcs, z, wfs = LUTSimFunction.apply(theta, sim, base, tracks, fields, n_events)
+            # the non-differentiable outputs were kept on the function's context
+            o = _last_others(adcs)
+            return dict(adcs=adcs, pixel_x=o["pixel_x"], pixel_y=o["pixel_y"], pixel_z=z, ticks=o["ticks"],
+                        hit_prob=o["hit_prob"], event=o["event"], hit_pixels=o["hit_pixels"],
+                        unique_pixels=o["unique_pixels"], wfs=wfs)
+        st = sim.forward(params, tracks, fields, n_events=n_events, want_wfs=True)
+        h = hits_dict(st.out)
+        return dict(adcs=h["adcs"], pixel_x=h["pixel_x"], pixel_y=h["pixel_y"], pixel_z=h["pixel_z"], ticks=h["ticks"],
+                    hit_prob=h["hit_prob"], event=h["event"], hit_pixels=h["hit_pixels"],
+                    unique_pixels=st.out["pixel_ids"][: st.max_rows], wfs=st.out["wfs"][: st.max_rows])
+
+
+def _last_others(t):
+    return t.grad_fn.others if t.grad_fn is not None else {}
+
+
+class LossStrategy:
+    def __init__(self, **kwargs):
+        self.kwargs = kwargs
+
+    def compute(self, params, prediction, target):
+        raise NotImplementedError
+
+
+class GenericLossStrategy(LossStrategy):
+    """adc -> charge, then ``loss_fn(params, Q, x, y, z, ticks, hit_prob, event, ref_Q, ...)``
+    (strategies.py:162-180)."""
+
+    def __init__(self, loss_fn, **kwargs):
+        super().__init__(**kwargs)
+        self.loss_fn = loss_fn
+
+    def compute(self, params, prediction, target):
+        Q = adc2charge(prediction["adcs"], params)
+        ref_Q = adc2charge(target["adcs"], params)
+        return self.loss_fn(params, Q, prediction["pixel_x"], prediction["pixel_y"], prediction["pixel_z"],
+                            prediction["ticks"], prediction["hit_prob"], prediction["event"], ref_Q, target["pixel_x"],
+                            target["pixel_y"], target["pixel_z"], target["ticks"], target["hit_prob"], target["event"],
+                            **self.kwargs)
