@@ -1,0 +1,224 @@
+"""Benchmark of the LUT simulation chain on one MI355X.
+
+Headline workload: one event of 200k track segments (500 tracks of 10-150 cm in a 30x30x30 cm
+drift volume, module0 geometry, 45x45x1950 LUT, 4 neighbour pixels, 5x5 diffusion bins, noise
+off).  A step = one forward pass (quench -> drift -> pixel rows -> waveforms -> self-trigger ->
+hits) over device-resident segments.  Prints one JSON line.
+
+Also measured in the same run (extra keys): forward+backward (7-parameter gradient) on the same
+event, the plain-forward finite-difference gradient baseline (15 forwards), the batched forward
+of 128 uneven events in one hipGraph replay, and the CPU baseline (the float32 NumPy restatement
+of the reference algorithm on a bounded sample, on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "larnd-sim-jax_amd"))
+
+HBM_PEAK_GBS = 8000.0
+LUT_BYTES = 45 * 45 * 1950 * 4
+
+
+def algorithmic_bytes_fwd(n_seg: int, rows: int, H: int = 10, nout: int = 2000) -> int:
+    """Compulsory HBM bytes of one forward: segment inputs (x, y, z, dE, dEdx, eventID as f32),
+    the raw LUT read once, the waveform rows written, and per row the pixel id plus the ADC and
+    tick of each of the H hit slots."""
+    return 24 * n_seg + LUT_BYTES + rows * nout * 4 + rows * (8 + H * 8)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(params, lut, tracks, fields, n_sample: int):
+    """The float32 NumPy restatement of the reference algorithm (materialise + bincount), single
+    thread, on the first n_sample segments; LUT template build excluded (as on the GPU)."""
+    from threadpoolctl import threadpool_limits
+
+    sys.path.insert(0, ROOT)
+    import numpy as np
+
+    from oracle import oracle as O
+
+    sub = tracks[:n_sample]
+    with threadpool_limits(limits=1):
+        tables = O.build_templates(params, lut)
+        t0 = time.perf_counter()
+        wfs, U, q = O.simulate_wfs(params, lut, sub, fields, tables=tables, dtype=np.float32)
+        O.simulate_stochastic(params, wfs, U)
+        dt = time.perf_counter() - t0
+    return dict(value=n_sample / dt, unit="segments/s", cores=1, kind="port",
+                sample=f"first {n_sample} segments of the 200k-segment event, float32 NumPy restatement "
+                       f"(materialised contributions + bincount), 1 thread, {dt:.1f} s",
+                host_cpu_count=os.cpu_count())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--waves", type=int, default=0, help="waves per row workgroup (0 = autotune)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-bwd", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=3000)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events shard across GPUs "
+              "with no exchange in the forward; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from larndsim_amd import consts, sim as S, synthetic
+
+    dev = torch.device("cuda")
+    params = consts.default_params(number_pix_neighbors=4, RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0)
+    lut = synthetic.make_lut(0)
+    log("[bench] generating the 200k-segment event")
+    tracks, fields = synthetic.workload("c2")
+    N = tracks.shape[0]
+    simr = S.Simulator(params, lut)
+    trk = torch.from_numpy(tracks).to(dev)
+    run = S.Runner(simr, params, trk, fields, n_events=1, want_wfs=True, want_tape=True)
+    P = run.rows
+    log(f"[bench] N={N} rows={P}")
+    if a.waves:
+        S.set_row_waves(a.waves)
+        waves, tune = a.waves, {}
+    else:
+        waves, tune = S.autotune_row_waves(run)
+    log(f"[bench] waves per row workgroup: {waves} (tuning ms: {tune})")
+
+    # ---- plain forward (the headline) ----
+    for _ in range(a.warmup):
+        run.forward()
+    torch.cuda.synchronize()
+    S.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run.forward()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    k_ms, k_n = S.timing_read()
+    S.timing(False)
+    ms_step = dt / a.steps * 1e3
+    seg_s = N / (dt / a.steps)
+    k_avg = k_ms / max(k_n, 1)
+    bytes_fwd = algorithmic_bytes_fwd(N, P)
+    achieved = bytes_fwd / (k_avg * 1e-3) / 1e9
+    log(f"[bench] forward {ms_step:.3f} ms/step, {seg_s:.3e} seg/s; waveform kernel {k_avg:.3f} ms")
+
+    extra = {}
+    # ---- forward + backward ----
+    if not a.no_bwd:
+        rng = np.random.default_rng(7)
+        d_adc = torch.from_numpy(rng.normal(size=(P * params.MAX_ADC_VALUES,)).astype(np.float32)).to(dev)
+        run.set_cotangents(d_adc)
+        for _ in range(a.warmup):
+            run.forward()
+            run.backward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            run.forward()
+            run.backward()
+        torch.cuda.synchronize()
+        dtb = (time.perf_counter() - t0) / a.steps
+        # finite-difference baseline: nominal + 2 x 7 perturbed plain forwards
+        fd_params = [params] + [params.replace(**{n: getattr(params, n) * (1 + s * 1e-3)})
+                                for n in consts.GRAD_PARAMS for s in (1, -1)]
+        t0 = time.perf_counter()
+        for _ in range(max(1, a.steps // 5)):
+            for pp in fd_params:
+                run.set_params(pp)
+                run.forward()
+        torch.cuda.synchronize()
+        dtfd = (time.perf_counter() - t0) / max(1, a.steps // 5)
+        run.set_params(params)
+        extra["fwd_bwd"] = dict(ms_per_step=dtb * 1e3, segments_per_s=N / dtb)
+        extra["fd_gradient_baseline"] = dict(forwards=len(fd_params), ms_per_gradient=dtfd * 1e3,
+                                             segments_per_s=N / dtfd, adjoint_speedup=dtfd / dtb)
+        log(f"[bench] fwd+bwd {dtb * 1e3:.3f} ms; FD gradient {dtfd * 1e3:.3f} ms (x{dtfd / dtb:.1f})")
+
+    # ---- batched forward: 128 uneven events, one graph replay ----
+    if not a.no_batched:
+        log("[bench] generating 128 events (log-uniform 2k-400k segments)")
+        bt, bf = synthetic.workload("c3")
+        NB = bt.shape[0]
+        btr = torch.from_numpy(bt).to(dev)
+        del bt
+        brun = S.Runner(simr, params, btr, bf, n_events=128, want_wfs=True)
+        brun.forward()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            brun.forward()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            brun.forward()
+        torch.cuda.synchronize()
+        for _ in range(2):
+            g.replay()
+        torch.cuda.synchronize()
+        bsteps = max(3, a.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(bsteps):
+            g.replay()
+        torch.cuda.synchronize()
+        dtg = (time.perf_counter() - t0) / bsteps
+        extra["batched"] = dict(events=128, segments=NB, rows=brun.rows, ms_per_step=dtg * 1e3,
+                                segments_per_s=NB / dtg, graph=True)
+        log(f"[bench] batched {NB} segments {dtg * 1e3:.2f} ms -> {NB / dtg:.3e} seg/s")
+        del brun, btr, g
+
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(params, lut, tracks, fields, a.cpu_sample)
+        log(f"[bench] CPU {cpu['value']:.1f} seg/s")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "rows_fwd_c2_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    line = {
+        "metric": "segments/s fwd & fwd+bwd (grad of 7 det params), 1 GPU, % of HBM roofline",
+        "value": seg_s,
+        "unit": "segments/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 index math)",
+        "data": "synthetic (seeded tracks and 45x45x1950 response LUT)",
+        "config": {"workload": "1 event, 200k segments (500 tracks, 10-150 cm), module0 geometry, "
+                               "45x45x1950 LUT, 4 neighbour pixels, 5x5 diffusion bins, noise off",
+                   "segments": N, "pixel_rows": P, "row_waves": waves},
+        "roofline": {"bound": "hbm", "kernel": "k_rows_fwd", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": bytes_fwd, "kernel_ms": k_avg},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,11 +153,18 @@ int lsim_fee_forward(lsim_ctx *ctx, const lsim_params *p, const float *wfs, cons
                      void *ws, size_t ws_bytes, const lsim_fwd_out *out, void *stream);
 
 /* Copy the per-segment records of the last prepare call (in main-pixel order) for inspection:
- * rec_out[n_seg * 80 bytes] = {int bx, by, k0, idx; float n_e, f, a, b, c, wx[5], wy[5], pad},
+ * rec_out[n_seg * 80 bytes] = {int k0, uxy; float n_e, f; int bx, by, idx; float a, b, c, wx[5], wy[5]}
+ * (uxy = (bx mod 10) | (by mod 10) << 16),
  * keys_out[n_seg] = cell key (>= number of cells for dropped segments), order_out[n_seg] =
  * input index of each record. */
 int lsim_segment_records(lsim_ctx *ctx, int64_t n_seg, int32_t n_events, void *ws, size_t ws_bytes, void *rec_out,
                          uint32_t *keys_out, uint32_t *order_out, void *stream);
+
+/* Measurement: while enabled, hipEvents are recorded on the launch stream around every launch
+ * of the waveform kernel; lsim_timing_read waits for them and returns the summed kernel time
+ * (ms) and the number of launches since lsim_timing(1).  Not for use under graph capture. */
+int lsim_timing(int enable);
+int lsim_timing_read(double *total_ms, int64_t *launches);
 
 /* Tuning: waves per pixel-row workgroup of the waveform kernel (1, 2, 4 or 8; default 4). */
 int lsim_set_row_waves(int waves);

@@ -38,10 +38,18 @@ struct Dev {
 
 // Per-segment record consumed by the waveform kernel, stored in main-pixel (cell) order.
 struct __align__(16) SegRec {
-  int bx, by, k0, idx;   // sub-bin indices, cathode tick, template index
-  float ne, f, la, lb;   // charge, fractional tick, Lagrange weights
-  float lc, wx[5];       // (wx: transverse weights along x)
-  float wy[5], pad;      // (wy: along y)
+  int k0;                // cathode tick (integer part of t0 / t_sampling)
+  int uxy;               // sub-bin position inside the main pixel: ux | uy << 16
+  float ne, f;           // charge, fractional tick (k0..f: all a neighbour stripe reads)
+  int bx, by, idx;       // sub-bin indices, template index
+  float la, lb, lc;      // Lagrange weights
+  float wx[5], wy[5];    // transverse weights along x and y
+};
+
+// The first 16 bytes of a SegRec: what the neighbour-stripe path reads.
+struct __align__(16) FarRec {
+  int k0, uxy;
+  float ne, f;
 };
 static_assert(sizeof(SegRec) == 80, "SegRec layout");
 
@@ -105,6 +113,14 @@ __device__ __forceinline__ void py_divmod(double a, double b, double& fl, double
 __device__ __forceinline__ int to_int_clamped(double v) {
   if (!(v > -1.0e9 && v < 1.0e9)) return -1000000000;
   return (int)v;
+}
+
+// LUT bin of a sub-bin position u (0 <= u < nsb) seen from a pixel `a` pitches away:
+// floor(|u - nsb/2 + 0.5 - nsb*a|) (sim_jax.py:435, 443), in integer arithmetic (nsb even).
+__device__ __forceinline__ int offset_bin(int u, int a, int nsb) {
+  int t = 2 * u - nsb + 1 - 2 * nsb * a;
+  t = t < 0 ? -t : t;
+  return (t - 1) >> 1;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -71,7 +71,7 @@ __global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, co
     r.wx[k] = k < d.nb ? (float)wx[k] : 0.f;
     r.wy[k] = k < d.nb ? (float)wy[k] : 0.f;
   }
-  r.pad = 0.f;
+  r.uxy = pymod_i(ph.bx, d.nsb) | (pymod_i(ph.by, d.nsb) << 16);
   rec[i] = r;
 }
 
@@ -117,6 +117,7 @@ __global__ void k_copy_count(const int* __restrict__ src, int64_t* __restrict__ 
 // ------------------------------------------------------------------------------------------
 
 struct Tables {
+  const float3* TM3;  // [nm][cm][cm][L]   (T_{m-1}, T_m, T_{m+1}) windows for the diffusion bins
   const float* TM;    // [nm][cm][cm][L]   main-bin template windows
   const float* TN;    // [lx][ly][L]       raw response window (neighbours)
   const float2* DM;   // [nm][cm][cm][nt]  (C[nt-L]-C[k], C[min(k+1,nt-1)]-C[k]) of template m
@@ -254,7 +255,8 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
 
 // Sum of the per-wave partial rows, waveform store, float64 cumulative charge into Q and Q0.
 template <int NT>
-__device__ void finish_row(const Dev& d, float* acc, int nwaves, double* Q, double* Q0, int r, const RowOut& out, double* shd) {
+__device__ void finish_row(const Dev& d, float* acc, int nwaves, int rstride, double* Q, double* Q0, int r, const RowOut& out,
+                           double* shd) {
   constexpr int TPER = kMaxOut / NT;
   const int t = threadIdx.x;
   float row[TPER];
@@ -264,7 +266,8 @@ __device__ void finish_row(const Dev& d, float* acc, int nwaves, double* Q, doub
   for (int u = 0; u < TPER; ++u) {
     const int o = t * TPER + u;
     float v = 0.f;
-    for (int w = 0; w < nwaves; ++w) v += acc[w * kMaxOut + o];
+    if (o < d.nout)
+      for (int w = 0; w < nwaves; ++w) v += acc[w * rstride + o];
     row[u] = v;
     if (o < d.nout) run += (double)v * d.ts;
     loc[u] = run;
@@ -282,172 +285,435 @@ __device__ void finish_row(const Dev& d, float* acc, int nwaves, double* Q, doub
 #pragma unroll
   for (int u = 0; u < TPER; ++u) {
     const int o = t * TPER + u;
-    Q[o] = ex + loc[u];
-    Q0[o] = ex + loc[u];
+    if (o < d.nout) {
+      Q[o] = ex + loc[u];
+      Q0[o] = ex + loc[u];
+    }
   }
   __syncthreads();
 }
 
-// Accumulate the contribution of segment record `R` (main pixel at offset (-a, -b) from this
-// row's pixel) into the wave's partial row.  The segment's stripes landing on this row share
+// Contributions of one segment to one pixel row.  The segment's stripes landing on the row share
 // one time window: V[j] = sum over stripes n_st * X_st[j], j < L, placed as (1-f) V at output
 // tick S+j and f V at S-1+j (S = nt - L - k0), plus the cumsum boundary correction Delta at
-// output ticks S-1 ((1-f) Delta) and S-2 (f Delta) (sim_jax.py:170-261).
-__device__ __forceinline__ void add_pair(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, int px, int py,
-                                         float* accw, float* stage) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int nsb = d.nsb, L = d.L;
-  const bool has_nb = (a != 0) || (b != 0);
-  const int ux = pymod_i(R.bx, nsb), uy = pymod_i(R.by, nsb);
-  int cxN = 0, cyN = 0;
-  if (has_nb) {
-    cxN = (int)fabsf((float)(ux - nsb / 2) + 0.5f - (float)(a * nsb));
-    cyN = (int)fabsf((float)(uy - nsb / 2) + 0.5f - (float)(b * nsb));
-  }
-  // diffusion bins whose pixel is this row's pixel
-  unsigned imask = 0, jmask = 0;
-  if (a >= -1 && a <= 1 && b >= -1 && b <= 1) {
-    for (int i = 0; i < d.nb; ++i) {
-      if (floordiv_i(R.bx + i - d.nbsym, nsb) == px) imask |= 1u << i;
-      if (floordiv_i(R.by + i - d.nbsym, nsb) == py) jmask |= 1u << i;
-    }
-  }
-  const bool has_bins = imask && jmask;
-  if (!has_nb && !has_bins) return;
+// output ticks S-1 ((1-f) Delta) and S-2 (f Delta) (sim_jax.py:170-261).  Lane l of chunk c
+// owns output offset m = 63*c + l - 2 and holds V[m]; V[m+1] comes from lane l+1.  Window ticks
+// are kept for 1 <= o <= nout-1, corrections for 0 <= o <= nout-2 (the reference's garbage
+// column takes the rest).  Every per-pair quantity is wave uniform (scalar registers).
+constexpr int kChunk = kWave - 1;
+constexpr int kRowPad = 3 * kChunk;  // slack after each wave's partial row for the interior store
 
-  const int cm = d.cm;
-  const int idx = R.idx;
-  const int64_t tstride = (int64_t)cm * cm * L;
-  const float* TM0 = tb.TM + (int64_t)(idx - 1) * tstride;
-  const float* TNp = tb.TN + ((int64_t)cxN * d.ly + cyN) * L;
-  const float ne = R.ne;
-
-  // window values
-  for (int j0 = 0; j0 < L; j0 += kWave) {
-    const int j = j0 + lane;
-    if (j < L) {
-      float v = 0.f;
-      if (has_nb) v = ne * TNp[j];
-      if (has_bins) {
-        for (int i = 0; i < d.nb; ++i) {
-          if (!((imask >> i) & 1)) continue;
-          const int cx = (int)fabsf((float)(pymod_i(R.bx + i - d.nbsym, nsb) - nsb / 2) + 0.5f);
-          const float nwx = ne * R.wx[i];
-          for (int jj = 0; jj < d.nb; ++jj) {
-            if (!((jmask >> jj) & 1)) continue;
-            const int cy = (int)fabsf((float)(pymod_i(R.by + jj - d.nbsym, nsb) - nsb / 2) + 0.5f);
-            const float* tp = TM0 + ((int64_t)cx * cm + cy) * L + j;
-            const float t1 = tp[0], t2 = tp[tstride], t3 = tp[2 * tstride];
-            v += (nwx * R.wy[jj]) * (t2 * R.lb + t1 * R.la + t3 * R.lc);
-          }
-        }
-      }
-      stage[2 + j] = v;
-    }
-  }
-  if (lane < 2) stage[lane] = 0.f;
-  if (lane == 2) stage[L + 2] = 0.f;
-
-  // boundary correction: one lane per stripe, then a fixed-order wave sum
-  float dl = 0.f;
-  const int k0 = R.k0;
-  const float f = R.f;
-  if (lane == 0 && has_nb) {
-    const float2 D = tb.DN[((int64_t)cxN * d.ly + cyN) * d.nt + k0];
-    dl = ne * (D.x - f * D.y);
-  } else if (lane >= 1 && lane <= kMaxNb * kMaxNb && has_bins) {
-    const int i = (lane - 1) / kMaxNb, jj = (lane - 1) % kMaxNb;
-    if (i < d.nb && jj < d.nb && ((imask >> i) & 1) && ((jmask >> jj) & 1)) {
-      const int cx = (int)fabsf((float)(pymod_i(R.bx + i - d.nbsym, nsb) - nsb / 2) + 0.5f);
-      const int cy = (int)fabsf((float)(pymod_i(R.by + jj - d.nbsym, nsb) - nsb / 2) + 0.5f);
-      const float2 D = tb.DM[(((int64_t)idx * cm + cx) * cm + cy) * d.nt + k0];
-      dl = (ne * R.wx[i] * R.wy[jj]) * (D.x - f * D.y);
-    }
-  }
-  const float delta = wave_sum(dl);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-  const int S = d.nt - L - k0;
-  const float omf = 1.f - f;
-  for (int k0l = 0; k0l < L + 2; k0l += kWave) {
-    const int k = k0l + lane;
-    if (k < L + 2) {
-      const int m = k - 2;
-      const int o = S + m;
-      // window ticks are kept for 1 <= o <= nout-1, corrections for 0 <= o <= nout-2
-      // (everything else goes to the reference's garbage column)
-      if (o >= 0 && o < d.nout) {
-        float val = 0.f;
-        if (o >= 1) val = omf * stage[m + 2] + f * stage[m + 3];
-        if (o <= d.nout - 2) {
-          if (m == -1) val += omf * delta;
-          if (m == -2) val += f * delta;
-        }
-        accw[o] += val;
-      }
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// lane i receives lane i+1's value (lane 63 receives 0): DPP wavefront shift, no LDS traffic
+__device__ __forceinline__ float lane_next(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130 /* wave_shl:1 */, 0xf, 0xf, false));
 }
 
-constexpr int kStage = 264;  // >= L + 3 for L <= 260
+// General store of one chunk with every bound checked (segments whose window reaches the ends).
+__device__ __forceinline__ void scatter_chunk(float* accw, int lane, int m, float v, float vn, float omf, float f,
+                                              float delta, int S, int L, int nout) {
+  const int o = S + m;
+  if (lane < kChunk && m < L && o >= 0 && o < nout) {
+    float val = 0.f;
+    if (o >= 1) val = omf * v + f * vn;
+    if (o <= nout - 2) {
+      if (m == -1) val += omf * delta;
+      if (m == -2) val += f * delta;
+    }
+    accw[o] += val;
+  }
+}
 
-template <int WAVES>
+// Store of a whole window held as three chunks (v0, v1, v2) of up to 63 outputs each, V = v
+// scaled by the charge: ne_omf = scale*(1-f), ne_f = scale*f.  `interior` windows
+// (2 <= S and S + L <= nout) need no per-lane tick checks: output m of chunk c goes to
+// accw[S + m] with m = 63 c + lane - 2.
+__device__ __forceinline__ void store_window(float* accw, int lane, float v0, float v1, float v2, float scale, float omf,
+                                             float f, float delta, int S, int L, int nout) {
+  const float n0 = lane_next(v0), n1 = lane_next(v1), n2 = lane_next(v2);
+  if (S >= 2 && S + L <= nout) {
+    // all three chunks in one go (disjoint addresses); lanes past the window add +0 to the
+    // row or to the kRowPad padding after it, so no per-chunk branch is needed
+    const float so = scale * omf, sf = scale * f;
+    float* p = accw + (S - 2) + lane;
+    float c0 = so * v0 + sf * n0;
+    c0 += (lane == 0) ? f * delta : ((lane == 1) ? omf * delta : 0.f);
+    const float c1 = (lane + kChunk - 2 < L) ? so * v1 + sf * n1 : 0.f;
+    const float c2 = (lane + 2 * kChunk - 2 < L) ? so * v2 + sf * n2 : 0.f;
+    if (lane < kChunk) {
+      const float a0 = p[0], a1 = p[kChunk], a2 = p[2 * kChunk];
+      p[0] = a0 + c0;
+      p[kChunk] = a1 + c1;
+      p[2 * kChunk] = a2 + c2;
+    }
+  } else {
+    scatter_chunk(accw, lane, lane - 2, scale * v0, scale * n0, omf, f, delta, S, L, nout);
+    scatter_chunk(accw, lane, lane - 2 + kChunk, scale * v1, scale * n1, omf, f, delta, S, L, nout);
+    scatter_chunk(accw, lane, lane - 2 + 2 * kChunk, scale * v2, scale * n2, omf, f, delta, S, L, nout);
+  }
+}
+
+// Neighbour stripe only (|a| or |b| > 1): raw response window, undiffused charge.  L <= 187.
+// Split into `far_load` (issue the window and correction loads) and `far_store` so a wave keeps
+// several independent pairs in flight.
+struct FarLoads {
+  float t0, t1, t2;
+  float2 D;
+};
+
+__device__ __forceinline__ void far_load(const Dev& d, const Tables& tb, const FarRec& R, int a, int b, int lane,
+                                         FarLoads& q) {
+  const int L = d.L;
+  const int cxN = offset_bin(R.uxy & 0xffff, a, d.nsb);
+  const int cyN = offset_bin(R.uxy >> 16, b, d.nsb);
+  const int sidx = cxN * d.ly + cyN;
+  const float* TNp = tb.TN + sidx * L;
+  q.D = tb.DN[sidx * d.nt + R.k0];
+  const int m0 = lane - 2;
+  q.t0 = (m0 >= 0) ? TNp[m0] : 0.f;
+  q.t1 = (m0 + kChunk < L) ? TNp[m0 + kChunk] : 0.f;
+  q.t2 = (m0 + 2 * kChunk < L) ? TNp[m0 + 2 * kChunk] : 0.f;
+}
+
+__device__ __forceinline__ void far_store(const Dev& d, const FarRec& R, const FarLoads& q, float* accw, int lane) {
+  const float f = R.f, omf = 1.f - f;
+  const float delta = R.ne * (q.D.x - f * q.D.y);
+  store_window(accw, lane, q.t0, q.t1, q.t2, R.ne, omf, f, delta, d.nt - d.L - R.k0, d.L, d.nout);
+}
+
+// |a|, |b| <= 1: the diffusion bins landing on this row (interpolated templates) plus the
+// neighbour stripe unless (a, b) = (0, 0).  TM3[idx][cx][cy][j] holds (T_{idx-1}, T_idx,
+// T_{idx+1}) so one 12-byte load serves the three Lagrange nodes.  Bin i of the segment lies
+// delta_i = floor((ux + i - SYM) / nsb) in {-1, 0, 1} pixels from the main pixel.
+template <int NB, bool CENTER>
+__device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, float* accw,
+                                          int lane) {
+  constexpr int SYM = (NB - 1) / 2;
+  const int nsb = d.nsb, L = d.L, cm = d.cm;
+  const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
+  int cxs[NB], cys[NB];
+  float wxs[NB], wys[NB];
+  bool anyx = false, anyy = false;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    int vx = ux + i - SYM, vy = uy + i - SYM;
+    const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
+    const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
+    vx -= dx * nsb;
+    vy -= dy * nsb;
+    cxs[i] = offset_bin(vx, 0, nsb);
+    cys[i] = offset_bin(vy, 0, nsb);
+    wxs[i] = (dx == a) ? R.ne * R.wx[i] : 0.f;  // zero weight: the bin lands on another pixel
+    wys[i] = (dy == b) ? R.wy[i] : 0.f;
+    anyx |= (dx == a);
+    anyy |= (dy == b);
+  }
+  const bool has_bins = anyx && anyy;
+  constexpr bool has_nb = !CENTER;
+  if (!has_nb && !has_bins) return;
+  const int idx = R.idx, k0 = R.k0;
+  const float f = R.f, omf = 1.f - f;
+  const float la = R.la, lb = R.lb, lc = R.lc;
+  const int m0 = lane - 2;
+  const bool ok0 = m0 >= 0, ok1 = m0 + kChunk < L, ok2 = m0 + 2 * kChunk < L;
+  float delta = 0.f;
+  float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+  if (has_nb) {
+    const int cxN = offset_bin(ux, a, nsb);
+    const int cyN = offset_bin(uy, b, nsb);
+    const int64_t sidx = (int64_t)cxN * d.ly + cyN;
+    const float2 D = tb.DN[sidx * d.nt + k0];
+    delta = R.ne * (D.x - f * D.y);
+    const float* TNp = tb.TN + sidx * L;
+    const float ne = R.ne;
+    if (ok0) v0 = ne * TNp[m0];
+    if (ok1) v1 = ne * TNp[m0 + kChunk];
+    if (ok2) v2 = ne * TNp[m0 + 2 * kChunk];
+  }
+  if (has_bins) {
+    const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * L;
+    const float3 z3 = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      // one uniform branch per row of bins: the row's 3 * NB loads overlap, rows do not (bounds
+      // the live registers); bins of the row that land elsewhere load but add zero
+      if (wxs[i] != 0.f) {
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) {
+          const float c = wxs[i] * wys[jj];
+          const int64_t coff = (int64_t)cxs[i] * cm + cys[jj];
+          if (c != 0.f) {
+            const float2 D = tb.DM[((int64_t)idx * cm * cm + coff) * d.nt + k0];
+            delta += c * (D.x - f * D.y);
+          }
+          const float3* tp = T3 + coff * L + m0;
+          const float3 q0 = ok0 ? tp[0] : z3;
+          const float3 q1 = ok1 ? tp[kChunk] : z3;
+          const float3 q2 = ok2 ? tp[2 * kChunk] : z3;
+          v0 += c * (q0.y * lb + q0.x * la + q0.z * lc);
+          v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
+          v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
+        }
+      }
+    }
+  }
+  store_window(accw, lane, v0, v1, v2, 1.f, omf, f, delta, d.nt - L - k0, L, d.nout);
+}
+
+// Any window length (L > 187): one chunk at a time, every bound checked.
+__device__ __forceinline__ void far_pair_any(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, float* accw,
+                                             int lane) {
+  const int nsb = d.nsb, L = d.L;
+  const int cxN = offset_bin(R.uxy & 0xffff, a, nsb);
+  const int cyN = offset_bin(R.uxy >> 16, b, nsb);
+  const int64_t sidx = (int64_t)cxN * d.ly + cyN;
+  const float* TNp = tb.TN + sidx * L;
+  const float2 D = tb.DN[sidx * d.nt + R.k0];
+  const float ne = R.ne, f = R.f, omf = 1.f - f;
+  const float delta = ne * (D.x - f * D.y);
+  const int S = d.nt - L - R.k0;
+  for (int base = -2; base < L; base += kChunk) {
+    const int m = base + lane;
+    const float v = (m >= 0 && m < L) ? ne * TNp[m] : 0.f;
+    const float vn = __shfl_down(v, 1, kWave);
+    scatter_chunk(accw, lane, m, v, vn, omf, f, delta, S, L, d.nout);
+  }
+}
+
+// Segment-parallel window of the diffusion bins that land on the segment's own (main) pixel:
+// vc[s][j] = sum_bins n_bin * (b T_idx + a T_{idx-1} + c T_{idx+1})[cx][cy][j] and its boundary
+// correction dc[s] (sim_jax.py:186-194, 236-247).  One wave per segment (sorted order); the
+// waveform kernel then adds these like a neighbour stripe.
+template <int NB>
+__global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* __restrict__ rec,
+                                                const uint32_t* __restrict__ keys, int64_t n, float* __restrict__ vc,
+                                                float* __restrict__ dc) {
+  constexpr int SYM = (NB - 1) / 2;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
+  if (s >= n || keys[s] >= (uint32_t)d.n_cells) return;
+  const SegRec R = rec[s];
+  const int nsb = d.nsb, L = d.L, cm = d.cm;
+  const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
+  int cxs[NB], cys[NB];
+  float wxs[NB], wys[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    int vx = ux + i - SYM, vy = uy + i - SYM;
+    const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
+    const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
+    cxs[i] = offset_bin(vx - dx * nsb, 0, nsb);
+    cys[i] = offset_bin(vy - dy * nsb, 0, nsb);
+    wxs[i] = (dx == 0) ? R.ne * R.wx[i] : 0.f;
+    wys[i] = (dy == 0) ? R.wy[i] : 0.f;
+  }
+  const float f = R.f, la = R.la, lb = R.lb, lc = R.lc;
+  const int idx = R.idx, k0 = R.k0;
+  const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * L;
+  float delta = 0.f;
+  for (int j0 = 0; j0 < L; j0 += 3 * kWave) {
+    const int j1 = j0 + lane, j2 = j1 + kWave, j3 = j2 + kWave;
+    const bool ok1 = j1 < L, ok2 = j2 < L, ok3 = j3 < L;
+    const float3 z3 = make_float3(0.f, 0.f, 0.f);
+    float v1 = 0.f, v2 = 0.f, v3 = 0.f;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      if (wxs[i] != 0.f) {
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) {
+          const float c = wxs[i] * wys[jj];
+          const int coff = cxs[i] * cm + cys[jj];
+          if (j0 == 0 && c != 0.f) {
+            const float2 D = tb.DM[((int64_t)idx * cm * cm + coff) * d.nt + k0];
+            delta += c * (D.x - f * D.y);
+          }
+          const float3* tp = T3 + (int64_t)coff * L;
+          const float3 q1 = ok1 ? tp[j1] : z3;
+          const float3 q2 = ok2 ? tp[j2] : z3;
+          const float3 q3 = ok3 ? tp[j3] : z3;
+          v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
+          v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
+          v3 += c * (q3.y * lb + q3.x * la + q3.z * lc);
+        }
+      }
+    }
+    float* o = vc + (int64_t)s * L;
+    if (ok1) o[j1] = v1;
+    if (ok2) o[j2] = v2;
+    if (ok3) o[j3] = v3;
+  }
+  if (lane == 0) dc[s] = delta;
+}
+
+// Center pair from the precomputed window: V = vc[s] (charge included), Delta = dc[s].
+__device__ __forceinline__ void center_load(const Dev& d, const float* __restrict__ vc, const float* __restrict__ dc,
+                                            int s, int lane, FarLoads& q) {
+  const int L = d.L;
+  const float* V = vc + (int64_t)s * L;
+  const int m0 = lane - 2;
+  q.t0 = (m0 >= 0) ? V[m0] : 0.f;
+  q.t1 = (m0 + kChunk < L) ? V[m0 + kChunk] : 0.f;
+  q.t2 = (m0 + 2 * kChunk < L) ? V[m0 + 2 * kChunk] : 0.f;
+  q.D = make_float2(dc[s], 0.f);
+}
+
+// Dynamic LDS bytes of the waveform kernel: the per-wave partial rows, reused afterwards for the
+// float64 cumulative charge (two copies).
+inline size_t rows_lds_bytes(int waves, int nout) {
+  const size_t acc = (size_t)waves * (nout + kRowPad) * 4, q = (size_t)2 * nout * 8;
+  return (acc > q ? acc : q);
+}
+
+constexpr int kMaxCells = 17 * 17;  // (2n+1)^2 for number_pix_neighbors <= 8
+constexpr int kXcd = 8;              // XCDs of an MI355X
+#ifndef LSIM_FAR_GROUP
+#define LSIM_FAR_GROUP 4
+#endif
+constexpr int kFarGroup = LSIM_FAR_GROUP;  // independent neighbour-only pairs in flight per wave
+
+template <int NB, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const int2* __restrict__ rng, const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
+                                                            const float* __restrict__ vc, const float* __restrict__ dc,
                                                             RowOut out) {
   constexpr int NT = WAVES * kWave;
-  constexpr int ACC_BYTES = WAVES * kMaxOut * 4;
-  constexpr int Q_BYTES = 2 * kMaxOut * 8;
-  constexpr int SM = ACC_BYTES > Q_BYTES ? ACC_BYTES : Q_BYTES;
-  __shared__ __align__(16) char smem[SM];
-  __shared__ float stage_all[WAVES][kStage];
+  extern __shared__ __align__(16) char smem[];
   __shared__ double shd[NT / kWave];
   __shared__ int shi[NT / kWave];
+  // this row's neighbourhood cells: far cells first, then the 3x3 near cells, in (a, b) order
+  __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
+  __shared__ int n_far_cells, n_center_end, n_cells_tot;
 
-  const int r = blockIdx.x;
-  const int P = *n_rows_dev;
-  if (r >= P || r >= max_rows) {
-    if (r < max_rows && threadIdx.x == 0) out.hit_cnt[r] = 0;
-    return;
-  }
+  // XCD-aware mapping: workgroups are dispatched round-robin over the XCDs, so workgroup i runs
+  // on XCD i % kXcd; give XCD x the contiguous band of rows [x*c, (x+1)*c) so the records and
+  // tables its rows share stay in that XCD's L2.
+  const int P0 = *n_rows_dev;
+  const int P = P0 < max_rows ? P0 : (int)max_rows;
+  if ((int64_t)blockIdx.x >= P && blockIdx.x < max_rows && threadIdx.x == 0) out.hit_cnt[blockIdx.x] = 0;
+  const int band = (P + kXcd - 1) / kXcd;
+  const int r = (blockIdx.x % kXcd) * band + blockIdx.x / kXcd;
+  if ((int)(blockIdx.x / kXcd) >= band || r >= P) return;
+  const int nout = d.nout;
   float* acc = (float*)smem;
-  const int t = threadIdx.x, wid = t / kWave;
-  for (int o = t; o < WAVES * kMaxOut; o += NT) acc[o] = 0.f;
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
+  const int rstride = nout + kRowPad;
+  for (int o = t; o < WAVES * rstride; o += NT) acc[o] = 0.f;
   const int cell = row_cell[r];
   const int px = cell % d.gx - d.halo;
   const int py = (cell / d.gx) % d.gy - d.halo;
   const int evp = cell / (d.gx * d.gy);  // ev * ntpc + plane
   const int plane = evp % d.ntpc;
-  if (t == 0) out.pixel_ids[r] = ((int64_t)evp * d.ny + py) * d.nx + px;
+  if (t == 0) {
+    out.pixel_ids[r] = ((int64_t)evp * d.ny + py) * d.nx + px;
+    int k = 0;
+    for (int pass = 0; pass < 3; ++pass) {  // far cells, the centre cell, the 8 side cells
+      for (int a = -d.nn; a <= d.nn; ++a) {
+        for (int b = -d.nn; b <= d.nn; ++b) {
+          const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
+          const int kind = !near ? 0 : ((a == 0 && b == 0) ? 1 : 2);
+          if (kind != pass) continue;
+          const int qx = px - a, qy = py - b;
+          if (qx < -d.halo || qx >= d.nx + d.halo || qy < -d.halo || qy >= d.ny + d.halo) continue;
+          const int64_t qc = ((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo);
+          const int2 rg = rng[qc];
+          if (rg.y <= rg.x) continue;
+          c_beg[k] = rg.x;
+          c_cum[k + 1] = rg.y - rg.x;
+          c_ab[k] = ((a + 64) << 8) | (b + 64);
+          ++k;
+        }
+      }
+      if (pass == 0) n_far_cells = k;
+      if (pass == 1) n_center_end = k;
+    }
+    n_cells_tot = k;
+    c_cum[0] = 0;
+    for (int i = 1; i <= k; ++i) c_cum[i] += c_cum[i - 1];
+  }
   __syncthreads();
 
-  float* accw = acc + wid * kMaxOut;
-  float* stage = stage_all[wid];
-  for (int a = -d.nn; a <= d.nn; ++a) {
-    const int qx = px - a;
-    if (qx < -d.halo || qx >= d.nx + d.halo) continue;
-    for (int b = -d.nn; b <= d.nn; ++b) {
-      const int qy = py - b;
-      if (qy < -d.halo || qy >= d.ny + d.halo) continue;
-      const int64_t qc = ((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo);
-      const int2 rg = rng[qc];
-      for (int s = rg.x + wid; s < rg.y; s += WAVES) {
-        const SegRec R = rec[s];
-        if (R.ne == 0.f) continue;
-        add_pair(d, tb, R, a, b, px, py, accw, stage);
+  float* accw = acc + wid * rstride;
+  const int nfc = n_far_cells, nct = n_cells_tot;
+  const int far_total = c_cum[nfc], tot = c_cum[nct];
+  // far pairs: cell by cell, kFarGroup independent segments per step (wave w takes the cell's
+  // segments w, w + WAVES, ...); then the near pairs
+  if (d.L <= 3 * kChunk - 2) {
+    for (int c = 0; c < nfc; ++c) {
+      const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
+      const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
+      const int ab = __builtin_amdgcn_readfirstlane(c_ab[c]);
+      const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+      const FarRec* fr = reinterpret_cast<const FarRec*>(rec + beg);
+      for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
+        FarRec R[kFarGroup];
+#pragma unroll
+        for (int g = 0; g < kFarGroup; ++g) {
+          const int j = k + g * WAVES;
+          R[g] = *reinterpret_cast<const FarRec*>(rec + beg + (j < cnt ? j : k));
+          if (j >= cnt) R[g].ne = 0.f;
+        }
+        FarLoads q[kFarGroup];
+#pragma unroll
+        for (int g = 0; g < kFarGroup; ++g) far_load(d, tb, R[g], a, b, lane, q[g]);
+#pragma unroll
+        for (int g = 0; g < kFarGroup; ++g)
+          if (R[g].ne != 0.f) far_store(d, R[g], q[g], accw, lane);
       }
+      (void)fr;
+    }
+  } else {
+    int cur = 0;
+    for (int i = wid; i < far_total; i += WAVES) {
+      while (i >= c_cum[cur + 1]) ++cur;
+      const int s = __builtin_amdgcn_readfirstlane(c_beg[cur] + (i - c_cum[cur]));
+      const int ab = __builtin_amdgcn_readfirstlane(c_ab[cur]);
+      const SegRec R = rec[s];
+      if (R.ne == 0.f) continue;
+      far_pair_any(d, tb, R, (ab >> 8) - 64, (ab & 255) - 64, accw, lane);
+    }
+  }
+  // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
+  for (int c = nfc; c < n_center_end; ++c) {
+    const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
+    const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
+    for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
+      FarRec R[kFarGroup];
+      int sg[kFarGroup];
+#pragma unroll
+      for (int g = 0; g < kFarGroup; ++g) {
+        const int j = k + g * WAVES;
+        sg[g] = beg + (j < cnt ? j : k);
+        R[g] = *reinterpret_cast<const FarRec*>(rec + sg[g]);
+        if (j >= cnt) R[g].ne = 0.f;
+      }
+      FarLoads q[kFarGroup];
+#pragma unroll
+      for (int g = 0; g < kFarGroup; ++g) center_load(d, vc, dc, sg[g], lane, q[g]);
+#pragma unroll
+      for (int g = 0; g < kFarGroup; ++g)
+        if (R[g].ne != 0.f)
+          store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, 1.f, 1.f - R[g].f, R[g].f, q[g].D.x, d.nt - d.L - R[g].k0,
+                       d.L, d.nout);
+    }
+  }
+  {
+    const int side_begin = c_cum[n_center_end];
+    int cur = n_center_end;
+    for (int i = side_begin + wid; i < tot; i += WAVES) {
+      while (i >= c_cum[cur + 1]) ++cur;
+      const int s = __builtin_amdgcn_readfirstlane(c_beg[cur] + (i - c_cum[cur]));
+      const int ab = __builtin_amdgcn_readfirstlane(c_ab[cur]);
+      const SegRec R = rec[s];
+      if (R.ne == 0.f) continue;
+      near_pair<NB, false>(d, tb, R, (ab >> 8) - 64, (ab & 255) - 64, accw, lane);
     }
   }
   __syncthreads();
   double* Q = (double*)smem;
-  double* Q0 = Q + kMaxOut;
-  finish_row<NT>(d, acc, WAVES, Q, Q0, r, out, shd);
+  double* Q0 = Q + nout;
+  finish_row<NT>(d, acc, WAVES, rstride, Q, Q0, r, out, shd);
   fee_row<NT>(d, Q, Q0, r, plane, out, shi);
 }
 
@@ -456,7 +722,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
 template <int NT>
 __global__ __launch_bounds__(NT) void k_fee_rows(Dev d, const float* __restrict__ wfs, const int64_t* __restrict__ pids,
                                                  int64_t n_rows, RowOut out) {
-  __shared__ __align__(16) char smem[2 * kMaxOut * 8 > kMaxOut * 4 ? 2 * kMaxOut * 8 : kMaxOut * 4];
+  __shared__ __align__(16) char smem[2 * kMaxOut * 8];
   __shared__ float accs[kMaxOut];
   __shared__ double shd[NT / kWave];
   __shared__ int shi[NT / kWave];
@@ -468,10 +734,10 @@ __global__ __launch_bounds__(NT) void k_fee_rows(Dev d, const float* __restrict_
   const int npl = d.nx * d.ny;
   const int plane = pid >= 0 ? (int)((pid / npl) % d.ntpc) : 0;
   double* Q = (double*)smem;
-  double* Q0 = Q + kMaxOut;
+  double* Q0 = Q + d.nout;
   RowOut o2 = out;
   o2.wfs = nullptr;
-  finish_row<NT>(d, accs, 1, Q, Q0, (int)r, o2, shd);
+  finish_row<NT>(d, accs, 1, d.nout, Q, Q0, (int)r, o2, shd);
   fee_row<NT>(d, Q, Q0, (int)r, plane, o2, shi);
   if (threadIdx.x == 0 && pid < 0) out.hit_cnt[r] = 0;
 }
@@ -610,6 +876,18 @@ __global__ __launch_bounds__(256) void k_lut_tables(Dev d, const float* __restri
       c += (double)win[j];
       CT[sbase * (L + 1) + j + 1] = c;
     }
+  }
+}
+
+// (T_{m-1}, T_m, T_{m+1}) windows for m in [1, nm-2]; other slabs are zero.
+__global__ void k_lut_tm3(Dev d, const float* __restrict__ TM, float3* __restrict__ TM3) {
+  const int64_t per = (int64_t)d.cm * d.cm * d.L;
+  const int64_t n = per * d.nm;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = i / per;
+    float3 v = make_float3(0.f, 0.f, 0.f);
+    if (m >= 1 && m <= d.nm - 2) v = make_float3(TM[i - per], TM[i], TM[i + per]);
+    TM3[i] = v;
   }
 }
 

@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "backward.h"
 
@@ -49,6 +51,7 @@ struct lsim_ctx {
   // LUT tables
   bool has_lut = false;
   int lx = 0, ly = 0, nt = 0;
+  float3* TM3 = nullptr;
   float* TM = nullptr;
   float* TN = nullptr;
   float2* DM = nullptr;
@@ -170,7 +173,8 @@ struct Ws {
   int* hit_off;
   int* scan_tmp;
   int* n_hits32;
-  int* sorted_sel;  // which ping-pong buffer holds the sorted keys (host written flag is not needed)
+  float* vc;  // [n][L] centre-pixel windows of each segment (sorted order)
+  float* dc;  // [n] their boundary corrections
   size_t bytes;
 };
 
@@ -193,9 +197,19 @@ Ws layout(void* base, int64_t n, int64_t n_cells) {
   w.hit_off = c.take<int>(n);
   w.scan_tmp = c.take<int>(scan_tiles(n) + 4);
   w.n_hits32 = c.take<int>(4);
-  w.sorted_sel = c.take<int>(4);
+  w.vc = nullptr;
+  w.dc = nullptr;
   w.bytes = c.off + 256;
   return w;
+}
+
+// The centre windows need L floats per segment; carved after the rest.
+void layout_center(Ws& w, void* base, int64_t n, int L) {
+  Carve c(base);
+  c.off = w.bytes;
+  w.vc = c.take<float>(n * L);
+  w.dc = c.take<float>(n);
+  w.bytes = c.off + 256;
 }
 
 int bits_for(int64_t v) {
@@ -213,15 +227,45 @@ void sorted_buffers(const Ws& w, int64_t n_cells, uint32_t*& keys, uint32_t*& va
   vals = (passes & 1) ? w.v1 : w.v0;
 }
 
+template <int NB, int WAVES>
+void launch_rows_nb(const Dev& d, const Tables& tb, const Ws& w, int64_t max_rows, const RowOut& ro, hipStream_t st) {
+  hipLaunchKernelGGL((k_rows_fwd<NB, WAVES>), dim3((unsigned)max_rows), dim3(WAVES * kWave),
+                     rows_lds_bytes(WAVES, d.nout), st, d, tb, w.rec, w.rng, w.row_cell, w.n_rows, max_rows, w.vc, w.dc,
+                     ro);
+}
+
 template <int WAVES>
 void launch_rows(const Dev& d, const Tables& tb, const Ws& w, int64_t max_rows, const RowOut& ro, hipStream_t st) {
-  hipLaunchKernelGGL((k_rows_fwd<WAVES>), dim3((unsigned)max_rows), dim3(WAVES * kWave), 0, st, d, tb, w.rec, w.rng,
-                     w.row_cell, w.n_rows, max_rows, ro);
+  if (d.nb == 5) launch_rows_nb<5, WAVES>(d, tb, w, max_rows, ro, st);
+  else if (d.nb == 3) launch_rows_nb<3, WAVES>(d, tb, w, max_rows, ro, st);
+  else launch_rows_nb<1, WAVES>(d, tb, w, max_rows, ro, st);
 }
 
 int g_row_waves = 4;
 
+// Optional timing of the waveform kernel: hipEvents recorded on the launch stream around every
+// k_rows_fwd launch while enabled (not for use under graph capture).
+struct RowTimer {
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t used = 0;
+} g_timer;
+
+void timer_begin(hipStream_t st, hipEvent_t* stop_out) {
+  *stop_out = nullptr;
+  if (!g_timer.on) return;
+  if (g_timer.used == g_timer.ev.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    g_timer.ev.emplace_back(a, b);
+  }
+  auto& pr = g_timer.ev[g_timer.used++];
+  (void)hipEventRecord(pr.first, st);
+  *stop_out = pr.second;
+}
+
 void free_tables(lsim_ctx* c) {
+  (void)hipFree(c->TM3);
   (void)hipFree(c->TM);
   (void)hipFree(c->TN);
   (void)hipFree(c->DM);
@@ -229,6 +273,7 @@ void free_tables(lsim_ctx* c) {
   (void)hipFree(c->CTM);
   (void)hipFree(c->CTN);
   c->TM = c->TN = nullptr;
+  c->TM3 = nullptr;
   c->DM = c->DN = nullptr;
   c->CTM = c->CTN = nullptr;
   c->has_lut = false;
@@ -250,6 +295,26 @@ int lsim_set_row_waves(int waves) {
   return LSIM_OK;
 }
 
+int lsim_timing(int enable) {
+  g_timer.on = enable != 0;
+  g_timer.used = 0;
+  return LSIM_OK;
+}
+
+int lsim_timing_read(double* total_ms, int64_t* launches) {
+  if (!total_ms || !launches) return fail(LSIM_E_ARG, "NULL argument");
+  double tot = 0.0;
+  for (size_t i = 0; i < g_timer.used; ++i) {
+    HIP_TRY(hipEventSynchronize(g_timer.ev[i].second));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, g_timer.ev[i].first, g_timer.ev[i].second));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = (int64_t)g_timer.used;
+  return LSIM_OK;
+}
+
 int lsim_ctx_create(lsim_ctx** out, const lsim_geometry* geo, const lsim_config* cfg) {
   if (!out || !geo || !cfg) return fail(LSIM_E_ARG, "NULL argument");
   *out = nullptr;
@@ -261,7 +326,7 @@ int lsim_ctx_create(lsim_ctx** out, const lsim_geometry* geo, const lsim_config*
   if (nsb < 2 || (nsb & 1)) return fail(LSIM_E_ARG, "nb_sampling_bins_per_pixel must be even and >= 2");
   if ((nb - 1) / 2 >= nsb) return fail(LSIM_E_ARG, "diffusion bins wider than a pixel are not supported");
   if (cfg->number_pix_neighbors < 0 || cfg->number_pix_neighbors > 8) return fail(LSIM_E_ARG, "number_pix_neighbors out of range");
-  if (cfg->signal_length < 1 || cfg->signal_length + 3 > kStage) return fail(LSIM_E_ARG, "signal_length out of range");
+  if (cfg->signal_length < 1 || cfg->signal_length > 1024) return fail(LSIM_E_ARG, "signal_length out of range");
   if (cfg->n_ticks < 4 || cfg->n_ticks - 1 > kMaxOut) return fail(LSIM_E_ARG, "n_ticks out of range (<= 2049)");
   if (cfg->max_adc_values < 1 || cfg->max_adc_values > kMaxH) return fail(LSIM_E_ARG, "max_adc_values out of range");
   if (cfg->n_templates < 3 || cfg->n_templates > LSIM_MAX_TEMPLATES) return fail(LSIM_E_ARG, "n_templates out of range");
@@ -301,6 +366,7 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
   const int64_t ms = (int64_t)k.n_templates * c->cm * c->cm, ns = (int64_t)nx * ny;
   const int L = k.signal_length;
   if (!c->has_lut) {
+    HIP_TRY(hipMalloc(&c->TM3, ms * L * sizeof(float3)));
     HIP_TRY(hipMalloc(&c->TM, ms * L * sizeof(float)));
     HIP_TRY(hipMalloc(&c->TN, ns * L * sizeof(float)));
     HIP_TRY(hipMalloc(&c->DM, ms * nt * sizeof(float2)));
@@ -313,6 +379,7 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
   if (rc) return rc;
   hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ms), dim3(256), 0, st, d, raw, 1, c->TM, c->DM, c->CTM);
   hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ns), dim3(256), 0, st, d, raw, 0, c->TN, c->DN, c->CTN);
+  hipLaunchKernelGGL(k_lut_tm3, dim3(1024), dim3(256), 0, st, d, c->TM, c->TM3);
   HIP_TRY(hipGetLastError());
   c->has_lut = true;
   return LSIM_OK;
@@ -324,6 +391,7 @@ int lsim_forward_query(lsim_ctx* c, int64_t n_seg, int32_t n_events, size_t* ws_
   int rc = make_dev(c, nullptr, n_events, d);
   if (rc) return rc;
   Ws w = layout(nullptr, n_seg, d.n_cells);
+  layout_center(w, nullptr, n_seg, d.L);
   *ws_bytes = w.bytes;
   if (max_rows_bound) {
     const int64_t plane_pixels = (int64_t)n_events * d.ntpc * d.nx * d.ny;
@@ -344,6 +412,7 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
   if (rc) return rc;
   const int64_t n = seg->n;
   Ws w = layout(ws, n, d.n_cells);
+  layout_center(w, ws, n, d.L);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small: need " + std::to_string(w.bytes) + " bytes");
   SegIn in = make_segin(seg);
   int* status = status_dev;
@@ -370,9 +439,9 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
   return LSIM_OK;
 }
 
-static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t max_rows, const lsim_fwd_out* out,
-                             hipStream_t st) {
-  Tables tb{c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
+static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_seg, int64_t max_rows,
+                             const lsim_fwd_out* out, hipStream_t st) {
+  Tables tb{c->TM3, c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
   RowOut ro;
   ro.wfs = out->wfs;
   ro.adc = out->adc;
@@ -381,7 +450,17 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t max
   ro.hit_cnt = w.hit_cnt;
   ro.pixel_ids = out->pixel_ids;
   hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, w.n_rows, out->n_rows, max_rows, out->status);
+  if (n_seg > 0) {
+    uint32_t *keys, *vals;
+    sorted_buffers(w, d.n_cells, keys, vals);
+    const unsigned g = (unsigned)((n_seg + 3) / 4);
+    if (d.nb == 5) hipLaunchKernelGGL(k_center<5>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc, w.dc);
+    else if (d.nb == 3) hipLaunchKernelGGL(k_center<3>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc, w.dc);
+    else hipLaunchKernelGGL(k_center<1>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc, w.dc);
+  }
   if (max_rows > 0) {
+    hipEvent_t stop_ev;
+    timer_begin(st, &stop_ev);
     switch (g_row_waves) {
       case 1: launch_rows<1>(d, tb, w, max_rows, ro, st); break;
       case 2: launch_rows<2>(d, tb, w, max_rows, ro, st); break;
@@ -389,6 +468,7 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t max
       default: launch_rows<4>(d, tb, w, max_rows, ro, st); break;
     }
     HIP_TRY(hipGetLastError());
+    if (stop_ev) HIP_TRY(hipEventRecord(stop_ev, st));
     HIP_TRY((exclusive_scan<int, int>(w.hit_cnt, w.hit_off, max_rows, w.scan_tmp, w.n_hits32, st)));
     if (out->hit_adc) {
       HitOut h{out->hit_adc, out->hit_x, out->hit_y, out->hit_z, out->hit_tick, out->hit_prob, out->hit_event,
@@ -425,10 +505,11 @@ int lsim_forward_rows(lsim_ctx* c, const lsim_params* p, int64_t n_seg, int32_t 
   rc = make_dev(c, p, n_events, d);
   if (rc) return rc;
   Ws w = layout(ws, n_seg, d.n_cells);
+  layout_center(w, ws, n_seg, d.L);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
   // rows never exceed segments; the per-row workspace buffers hold max(n, 1) entries
   if (max_rows > (n_seg > 1 ? n_seg : 1)) max_rows = n_seg > 1 ? n_seg : 1;
-  return forward_rows_impl(c, d, w, max_rows, out, (hipStream_t)stream);
+  return forward_rows_impl(c, d, w, n_seg, max_rows, out, (hipStream_t)stream);
 }
 
 int lsim_forward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws, size_t ws_bytes,
@@ -547,7 +628,7 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
   double* dz = cv.take<double>(max_rows);
   const int64_t nblk = (n + 255) / 256;
   double* partial = cv.take<double>(nblk * kNGrad);
-  Tables tb{c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
+  Tables tb{c->TM3, c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
   SegIn in = make_segin(seg);
   uint32_t *keys, *vals;
   sorted_buffers(w, d.n_cells, keys, vals);

@@ -375,3 +375,101 @@ def simulate_stochastic(params, wfs, unique_pixels, rngseed=0):
     out = sim.fee(params, torch.as_tensor(wfs), torch.as_tensor(unique_pixels))
     h = hits_dict(out)
     return (h["adcs"], h["pixel_x"], h["pixel_y"], h["pixel_z"], h["ticks"], h["hit_prob"], h["event"], h["hit_pixels"])
+
+
+# ------------------------------------------------------------------------------------------
+# preallocated runner: fixed buffers, one stream-ordered library call per step (capturable)
+# ------------------------------------------------------------------------------------------
+
+class Runner:
+    """Forward (and optionally backward) over fixed device buffers.
+
+    Construction does one two-phase forward to learn the row count (times ``row_margin``) and
+    allocates every buffer; ``forward()`` / ``backward()`` then make exactly one library call each
+    on the current stream with no allocation and no host synchronisation, so a step can be
+    captured into a hipGraph (``torch.cuda.CUDAGraph``) and replayed.
+    """
+
+    def __init__(self, sim: Simulator, params, tracks, fields, n_events=None, want_wfs=True, want_tape=False,
+                 row_margin=1.0):
+        _check_noise(params)
+        self.sim, self.lib = sim, sim.lib
+        self.fields = fields
+        self.tracks, self.seg = sim.segments(tracks, fields)
+        self.n_events = infer_n_events(self.tracks, fields) if n_events is None else n_events
+        self.set_params(params)
+        st = sim.forward(params, self.tracks, fields, n_events=self.n_events, want_wfs=False)
+        self.rows = st.max_rows
+        self.max_rows = max(int(self.rows * row_margin), 1)
+        self.ws = st.ws
+        self.out = sim.alloc_out(self.max_rows, want_wfs, want_tape)
+        self.oc = sim.out_struct(self.out)
+        self.bws = None
+        self.grad = torch.zeros(7, dtype=torch.float64, device=sim.device)
+        self.d_adc = None
+        self.d_z = None
+
+    def set_params(self, params):
+        self.params = params
+        self.pc = params_struct(params)
+
+    def forward(self):
+        _lib.check(self.lib.lsim_forward(self.sim._ctx, C.byref(self.pc), C.byref(self.seg), self.n_events,
+                                         C.c_void_p(self.ws.data_ptr()), self.ws.numel(), self.max_rows,
+                                         C.byref(self.oc), self.sim._stream()))
+
+    def set_cotangents(self, d_adc, d_z=None):
+        n = self.max_rows * self.sim.H
+        self.d_adc = torch.zeros(n, dtype=torch.float32, device=self.sim.device)
+        self.d_adc[: d_adc.numel()].copy_(d_adc.reshape(-1)[:n])
+        if d_z is not None:
+            self.d_z = torch.zeros(n, dtype=torch.float32, device=self.sim.device)
+            self.d_z[: d_z.numel()].copy_(d_z.reshape(-1)[:n])
+        sz = C.c_size_t()
+        _lib.check(self.lib.lsim_backward_query(self.sim._ctx, self.seg.n, self.max_rows, C.byref(sz)))
+        self.bws = torch.empty(int(sz.value), dtype=torch.uint8, device=self.sim.device)
+
+    def backward(self):
+        dz = C.c_void_p(self.d_z.data_ptr()) if self.d_z is not None else None
+        _lib.check(self.lib.lsim_backward(self.sim._ctx, C.byref(self.pc), C.byref(self.seg), self.n_events,
+                                          C.c_void_p(self.ws.data_ptr()), self.ws.numel(), self.max_rows,
+                                          C.byref(self.oc), C.c_void_p(self.d_adc.data_ptr()), dz,
+                                          C.c_void_p(self.bws.data_ptr()), self.bws.numel(),
+                                          C.c_void_p(self.grad.data_ptr()), self.sim._stream()))
+
+
+def timing(enable: bool):
+    lib = _lib.load()
+    _lib.check(lib.lsim_timing(1 if enable else 0))
+
+
+def timing_read():
+    lib = _lib.load()
+    ms = C.c_double()
+    n = C.c_int64()
+    _lib.check(lib.lsim_timing_read(C.byref(ms), C.byref(n)))
+    return float(ms.value), int(n.value)
+
+
+def set_row_waves(w: int):
+    _lib.check(_lib.load().lsim_set_row_waves(int(w)))
+
+
+def autotune_row_waves(runner: Runner, candidates=(1, 2, 4, 8), reps=3):
+    """Pick the waves-per-row-workgroup setting with the shortest waveform-kernel time."""
+    best, best_ms = None, None
+    res = {}
+    for w in candidates:
+        set_row_waves(w)
+        runner.forward()
+        torch.cuda.synchronize()
+        timing(True)
+        for _ in range(reps):
+            runner.forward()
+        ms, n = timing_read()
+        timing(False)
+        res[w] = ms / max(n, 1)
+        if best_ms is None or res[w] < best_ms:
+            best, best_ms = w, res[w]
+    set_row_waves(best)
+    return best, res

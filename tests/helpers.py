@@ -1,8 +1,8 @@
 """Shared comparison helpers for the parity tests."""
 import numpy as np
 
-REC_DTYPE = np.dtype([("bx", "<i4"), ("by", "<i4"), ("k0", "<i4"), ("idx", "<i4"), ("ne", "<f4"), ("f", "<f4"),
-                      ("la", "<f4"), ("lb", "<f4"), ("lc", "<f4"), ("wx", "<f4", 5), ("wy", "<f4", 5), ("pad", "<f4")])
+REC_DTYPE = np.dtype([("k0", "<i4"), ("uxy", "<i4"), ("ne", "<f4"), ("f", "<f4"), ("bx", "<i4"), ("by", "<i4"),
+                      ("idx", "<i4"), ("la", "<f4"), ("lb", "<f4"), ("lc", "<f4"), ("wx", "<f4", 5), ("wy", "<f4", 5)])
 
 
 def wfs_error(gpu, ref):

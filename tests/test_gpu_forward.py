@@ -49,6 +49,8 @@ def test_segment_records(sim, gpu_c1, oracle_c1):
     np.testing.assert_allclose(r["f"], (q["tau"] - k0)[o], atol=1e-6)
     for a, b in (("la", "la"), ("lb", "lb"), ("lc", "lc")):
         np.testing.assert_allclose(r[a], q[b][o], rtol=1e-5, atol=1e-6)
+    assert np.array_equal(r["uxy"] & 0xFFFF, np.remainder(q["bx"][o], 10))
+    assert np.array_equal(r["uxy"] >> 16, np.remainder(q["by"][o], 10))
     np.testing.assert_allclose(r["wx"], q["wx"][o], atol=1e-7)
     np.testing.assert_allclose(r["wy"], q["wy"][o], atol=1e-7)
 
