@@ -64,16 +64,28 @@ __global__ void k_fee_adjoint(Dev d, const FeeTape* __restrict__ tape, const flo
       ++n;
     }
   }
-  Pieces pc;
-  pc.n = n;
-  pc.pad[0] = pc.pad[1] = pc.pad[2] = 0;
+  // step values G on (pos[q-1], pos[q]]; keep only the steps where G changes, drop a zero tail
+  float cv[2 * kMaxH];
   double suf = 0.0;
   for (int q = n - 1; q >= 0; --q) {
     suf += val[q];
-    pc.x[q] = pos[q];
-    pc.c[q] = (float)(suf * d.ts);
+    cv[q] = (float)(suf * d.ts);
   }
-  for (int q = n; q < 2 * kMaxH; ++q) {
+  Pieces pc;
+  int m = 0;
+  for (int q = 0; q < n; ++q) {
+    if (m > 0 && cv[q] == pc.c[m - 1]) {
+      pc.x[m - 1] = pos[q];  // same value: extend the previous piece
+    } else {
+      pc.x[m] = pos[q];
+      pc.c[m] = cv[q];
+      ++m;
+    }
+  }
+  while (m > 0 && pc.c[m - 1] == 0.f) --m;
+  pc.n = m;
+  pc.pad[0] = pc.pad[1] = pc.pad[2] = 0;
+  for (int q = m; q < 2 * kMaxH; ++q) {
     pc.x[q] = 0x7fffffff;
     pc.c[q] = 0.f;
   }
@@ -88,22 +100,25 @@ __global__ void k_fee_adjoint(Dev d, const FeeTape* __restrict__ tape, const flo
   dz_part[r] = dz;  // d loss / d vdrift_eff through the hit z positions
 }
 
-// G(o) of a row at one output tick (0 outside [lo, 1999] validity handled by the caller).
-__device__ __forceinline__ double g_at(const Pieces& pc, int o) {
-  for (int q = 0; q < pc.n; ++q)
-    if (pc.x[q] >= o) return (double)pc.c[q];
+// G(o) of a row: c[q] for the first piece with x[q] >= o (0 past the last piece).
+__device__ __forceinline__ double g_at(const Pieces* __restrict__ pc, int n, int q0, int o) {
+  for (int q = q0; q < n; ++q)
+    if (pc->x[q] >= o) return (double)pc->c[q];
   return 0.0;
 }
 
-// sum_{o=lo..hi} G(o) * X[o - S + sh] using the window prefix sums CX (L+1 entries).
-__device__ __forceinline__ double g_dot(const Pieces& pc, const double* __restrict__ CX, int lo, int hi, int S, int sh) {
+// sum_{o=lo..hi} G(o) * X[o - S + sh] using the window prefix sums CX (L+1 entries); pieces
+// before q0 end below lo.
+__device__ __forceinline__ double g_dot(const Pieces* __restrict__ pc, int n, int q0, const double* __restrict__ CX, int lo,
+                                        int hi, int S, int sh) {
   double acc = 0.0;
-  int prev = -1;  // pieces cover (prev, x_q]
-  for (int q = 0; q < pc.n && prev < hi; ++q) {
+  int prev = q0 > 0 ? pc->x[q0 - 1] : -1;  // pieces cover (prev, x_q]
+  for (int q = q0; q < n && prev < hi; ++q) {
+    const int xq = pc->x[q];
     const int a = (prev + 1) > lo ? (prev + 1) : lo;
-    const int b = pc.x[q] < hi ? pc.x[q] : hi;
-    if (a <= b) acc += (double)pc.c[q] * (CX[b - S + sh + 1] - CX[a - S + sh]);
-    prev = pc.x[q];
+    const int b = xq < hi ? xq : hi;
+    if (a <= b) acc += (double)pc->c[q] * (CX[b - S + sh + 1] - CX[a - S + sh]);
+    prev = xq;
   }
   return acc;
 }
@@ -163,19 +178,24 @@ __global__ __launch_bounds__(256) void k_seg_adjoint(Dev d, SegIn in, Tables tb,
           }
           const bool has_bins = imask && jmask;
           if (!has_nb && !has_bins) continue;
-          const Pieces pc = pieces[r];
-          if (pc.n == 0) continue;
+          const Pieces* pc = pieces + r;
+          const int np = pc->n;
+          if (np == 0) continue;
+          // first piece reaching the earliest tick this segment touches (S - 2)
+          int q0 = 0;
+          while (q0 < np && pc->x[q0] < S - 2) ++q0;
+          if (q0 == np) continue;
           // correction cotangents: (1-f) Delta at o = S-1, f Delta at o = S-2 (valid for o >= 0)
-          const double gc0 = (S - 1 >= 0 && S - 1 <= nout - 2) ? g_at(pc, S - 1) : 0.0;
-          const double gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? g_at(pc, S - 2) : 0.0;
+          const double gc0 = (S - 1 >= 0 && S - 1 <= nout - 2) ? g_at(pc, np, q0, S - 1) : 0.0;
+          const double gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? g_at(pc, np, q0, S - 2) : 0.0;
           const double gc = (1.0 - f) * gc0 + f * gc1;
           if (has_nb) {
             const int cxN = (int)fabsf((float)(ux - nsb / 2) + 0.5f - (float)(a * nsb));
             const int cyN = (int)fabsf((float)(uy - nsb / 2) + 0.5f - (float)(b * nsb));
             const int64_t sidx = (int64_t)cxN * d.ly + cyN;
             const double* CX = tb.CTN + sidx * (L + 1);
-            const double y1 = g_dot(pc, CX, lo1, hi1, S, 0);
-            const double y0 = g_dot(pc, CX, lo0, hi0, S, 1);
+            const double y1 = g_dot(pc, np, q0, CX, lo1, hi1, S, 0);
+            const double y0 = g_dot(pc, np, q0, CX, lo0, hi0, S, 1);
             const float2 D = tb.DN[sidx * nt + ph.k0];
             const double corr = (double)D.x - f * (double)D.y;
             gN += (1.0 - f) * y1 + f * y0 + gc * corr;
@@ -192,9 +212,9 @@ __global__ __launch_bounds__(256) void k_seg_adjoint(Dev d, SegIn in, Tables tb,
                 const double* C1 = tb.CTM + base;
                 const double* C2 = C1 + tstride;
                 const double* C3 = C2 + tstride;
-                const double y1a = g_dot(pc, C1, lo1, hi1, S, 0), y0a = g_dot(pc, C1, lo0, hi0, S, 1);
-                const double y1b = g_dot(pc, C2, lo1, hi1, S, 0), y0b = g_dot(pc, C2, lo0, hi0, S, 1);
-                const double y1c = g_dot(pc, C3, lo1, hi1, S, 0), y0c = g_dot(pc, C3, lo0, hi0, S, 1);
+                const double y1a = g_dot(pc, np, q0, C1, lo1, hi1, S, 0), y0a = g_dot(pc, np, q0, C1, lo0, hi0, S, 1);
+                const double y1b = g_dot(pc, np, q0, C2, lo1, hi1, S, 0), y0b = g_dot(pc, np, q0, C2, lo0, hi0, S, 1);
+                const double y1c = g_dot(pc, np, q0, C3, lo1, hi1, S, 0), y0c = g_dot(pc, np, q0, C3, lo0, hi0, S, 1);
                 const double y1 = lb * y1b + la * y1a + lc * y1c;
                 const double y0 = lb * y0b + la * y0a + lc * y0c;
                 const float2 D = tb.DM[(((int64_t)ph.idx * cm + cx) * cm + cy) * nt + ph.k0];
