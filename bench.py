@@ -7,8 +7,14 @@ hits) over device-resident segments.  Prints one JSON line.
 
 Also measured in the same run (extra keys): forward+backward (7-parameter gradient) on the same
 event, the plain-forward finite-difference gradient baseline (15 forwards), the batched forward
-of 128 uneven events in one hipGraph replay, and the CPU baseline (the float32 NumPy restatement
-of the reference algorithm on a bounded sample, on the host cores).
+of 128 uneven events in one hipGraph replay, the batched forward+backward of 32 uneven events in
+one hipGraph replay against the per-event forward-then-finite-difference gradient, and the CPU
+baseline (the float32 NumPy restatement of the reference algorithm on a bounded sample, on the
+host cores).
+
+``--copy-calibrate`` only runs a plain 256 MiB device copy five times: profiled with the same
+``rocprofv3 --pmc FETCH_SIZE`` / ``WRITE_SIZE`` passes it gives the counter-to-bytes factor
+used to turn the waveform kernel's counters into HBM bytes per launch.
 """
 from __future__ import annotations
 
@@ -68,7 +74,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-bwd", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=3000)
+    ap.add_argument("--no-fit", action="store_true", help="skip the batched forward+backward of 32 events")
+    ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--copy-calibrate", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events shard across GPUs "
@@ -77,6 +85,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if a.copy_calibrate:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 5}))
+        return
 
     from larndsim_amd import consts, sim as S, synthetic
 
@@ -182,6 +199,64 @@ def main():
                                 segments_per_s=NB / dtg, graph=True)
         log(f"[bench] batched {NB} segments {dtg * 1e3:.2f} ms -> {NB / dtg:.3e} seg/s")
         del brun, btr, g
+
+    # ---- batched forward + backward: 32 uneven events, one graph replay, vs per-event FD ----
+    if not a.no_fit:
+        log("[bench] generating 32 events (20k-200k segments)")
+        ft, ff = synthetic.workload("c5")
+        NF = ft.shape[0]
+        ev = ft[:, ff.index("eventID")].astype(np.int64)
+        frun = S.Runner(simr, params, torch.from_numpy(ft).to(dev), ff, n_events=32, want_wfs=False, want_tape=True)
+        rng = np.random.default_rng(9)
+        frun.set_cotangents(torch.from_numpy(rng.normal(size=(frun.max_rows * params.MAX_ADC_VALUES,))
+                                             .astype(np.float32)).to(dev))
+        frun.forward()
+        frun.backward()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            frun.forward()
+            frun.backward()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            frun.forward()
+            frun.backward()
+        torch.cuda.synchronize()
+        for _ in range(2):
+            g.replay()
+        torch.cuda.synchronize()
+        fsteps = max(3, a.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(fsteps):
+            g.replay()
+        torch.cuda.synchronize()
+        dtf = (time.perf_counter() - t0) / fsteps
+        del g
+        # baseline: per event, 1 + 14 plain forwards (central differences, h = 1e-3 relative)
+        t_events = 0.0
+        for e in range(32):
+            sub = ft[ev == e].copy()
+            sub[:, ff.index("eventID")] = 0
+            erun = S.Runner(simr, params, torch.from_numpy(sub).to(dev), ff, n_events=1, want_wfs=True)
+            erun.forward()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                erun.forward()
+            torch.cuda.synchronize()
+            t_events += (time.perf_counter() - t0) / 3
+            del erun
+        t_fd = 15 * t_events
+        extra["batched_fwd_bwd"] = dict(events=32, segments=NF, rows=frun.rows, ms_per_step=dtf * 1e3,
+                                        segments_per_s=NF / dtf, graph=True, wfs_stored=False)
+        extra["per_event_fd_baseline"] = dict(forwards_per_event=15, ms_per_gradient=t_fd * 1e3,
+                                              segments_per_s=NF / t_fd, adjoint_speedup=t_fd / dtf)
+        log(f"[bench] batched fwd+bwd {NF} segments {dtf * 1e3:.2f} ms -> {NF / dtf:.3e} seg/s; "
+            f"per-event FD {t_fd * 1e3:.1f} ms (x{t_fd / dtf:.1f})")
+        del frun
 
     cpu = None
     if not a.no_cpu:

@@ -585,14 +585,19 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   __shared__ int n_far_cells, n_center_end, n_cells_tot;
 
   // XCD-aware mapping: workgroups are dispatched round-robin over the XCDs, so workgroup i runs
-  // on XCD i % kXcd; give XCD x the contiguous band of rows [x*c, (x+1)*c) so the records and
-  // tables its rows share stay in that XCD's L2.
+  // on XCD i % kXcd.  Workgroups 0..P-1 map one-to-one onto rows 0..P-1: XCD x receives the
+  // q + (x < rem) workgroups with i % kXcd == x (q = P / kXcd, rem = P % kXcd) and gives them
+  // the contiguous band of rows starting at x*q + min(x, rem), so the records and tables its
+  // rows share stay in that XCD's L2.
   const int P0 = *n_rows_dev;
   const int P = P0 < max_rows ? P0 : (int)max_rows;
-  if ((int64_t)blockIdx.x >= P && blockIdx.x < max_rows && threadIdx.x == 0) out.hit_cnt[blockIdx.x] = 0;
-  const int band = (P + kXcd - 1) / kXcd;
-  const int r = (blockIdx.x % kXcd) * band + blockIdx.x / kXcd;
-  if ((int)(blockIdx.x / kXcd) >= band || r >= P) return;
+  if ((int64_t)blockIdx.x >= P) {
+    if (blockIdx.x < max_rows && threadIdx.x == 0) out.hit_cnt[blockIdx.x] = 0;
+    return;
+  }
+  const int xcd = (int)(blockIdx.x % kXcd);
+  const int q8 = P / kXcd, rem = P % kXcd;
+  const int r = xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
   const int nout = d.nout;
   float* acc = (float*)smem;
   const int t = threadIdx.x;
@@ -769,6 +774,7 @@ __global__ void k_hits(Dev d, const int64_t* __restrict__ pids, const float* __r
   int before = 0;
   for (int kk = 0; kk < k; ++kk) before += (ticks[r * H + kk] < d.nout - 3) ? 1 : 0;
   const int64_t pos = (int64_t)hit_off[r] + before;
+  if (pos < 0 || pos >= P * H) return;  // offsets come from a scan of per-row counts <= H
   const int npl = d.nx * d.ny;
   const int px = (int)(pid % d.nx), py = (int)((pid / d.nx) % d.ny);
   const int plane = (int)((pid / npl) % d.ntpc);
