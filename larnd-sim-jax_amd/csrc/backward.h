@@ -6,11 +6,12 @@
 // Q^(k)[e_k]; pass k then sets Q^(k+1) = max(Q^(k) - Q^(k)[g_k], 0).  Reverse mode over the 2H
 // tracked points gives the adjoint of Q^(0), and because Q^(0) is the cumulative sum of
 // t_sampling * wfs, d loss / d wfs is piecewise constant with at most 2H steps.
-// Stage 2 (one thread per segment): for every pixel row the segment reaches, inner products of
-// that step function with the segment's template windows come from window prefix sums, which
-// yields d loss / d (charge, transverse width, fractional tick, longitudinal width) of the
-// segment; the chain rule to the parameters follows and a fixed-order block reduction writes
-// per-block partial gradients.
+// Stage 2 (one workgroup per pixel row, like the forward's waveform kernel): for every segment
+// whose stripes reach the row, inner products of the row's step function with the segment's
+// template windows come from window prefix sums; they give d loss / d (charge, transverse width,
+// fractional tick, longitudinal width) of the pair, which the per-segment chain factors
+// (k_adj_records) turn into the 7 gradients; a fixed-order block reduction writes per-row
+// partial gradients.
 // Stage 3: fixed-order reduction of the partials (and of the hit-z term) to 7 values.
 #pragma once
 #include "forward.h"
@@ -18,6 +19,10 @@
 namespace lsim {
 
 constexpr int kNGrad = 7;
+#ifndef LSIM_ADJ_WAVES
+#define LSIM_ADJ_WAVES 4
+#endif
+constexpr int kAdjWaves = LSIM_ADJ_WAVES;  // waves per pixel-row workgroup of the adjoint
 
 __global__ void k_fee_adjoint(Dev d, const FeeTape* __restrict__ tape, const float* __restrict__ d_adc,
                               const float* __restrict__ d_z, const int64_t* __restrict__ n_rows_dev, int64_t max_rows,
@@ -100,170 +105,258 @@ __global__ void k_fee_adjoint(Dev d, const FeeTape* __restrict__ tape, const flo
   dz_part[r] = dz;  // d loss / d vdrift_eff through the hit z positions
 }
 
-// G(o) of a row: c[q] for the first piece with x[q] >= o (0 past the last piece).
-__device__ __forceinline__ double g_at(const Pieces* __restrict__ pc, int n, int q0, int o) {
-  for (int q = q0; q < n; ++q)
-    if (pc->x[q] >= o) return (double)pc->c[q];
-  return 0.0;
+// Per-segment factors of the chain rule, in the same (sorted) order as the SegRec records.
+struct __align__(16) AdjRec {
+  double fN[4];          // d n_e / d (Ab, kb, eField, lifetime)
+  double fS[2];          // d sigma_T / d (eField, tran_diff)
+  double fL[2];          // d lambda / d (eField, long_diff)
+  double fF;             // d f / d eField
+  double dl[3];          // d (a, b, c) / d lambda of the Lagrange weights
+  float dwx[5], dwy[5];  // d w / d sigma_T of the transverse weights
+  float pad[2];
+};
+static_assert(sizeof(AdjRec) == 144, "AdjRec layout");
+
+// One thread per record: recompute the segment's physics (float64) and store the chain factors.
+__global__ __launch_bounds__(256) void k_adj_records(Dev d, SegIn in, const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ vals, int64_t n,
+                                                     AdjRec* __restrict__ adj) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || keys[s] >= (uint32_t)d.n_cells) return;
+  Phys ph;
+  seg_physics(d, in, vals[s], ph);
+  double wx[kMaxNb], wy[kMaxNb], dwx[kMaxNb], dwy[kMaxNb];
+  tran_weights(d, ph.x0, ph.st, wx, dwx);
+  tran_weights(d, ph.y0, ph.st, wy, dwy);
+  const double v = d.v, dv = d.dv;
+  const double att = exp(-ph.td / d.lifetime) * (double)ph.mask;
+  AdjRec A;
+  A.fN[0] = ph.dq_dAb * att;
+  A.fN[1] = ph.dq_dkb * att;
+  A.fN[2] = ph.dq_dE_rec * att + ph.ne * ph.td * dv / (v * d.lifetime);
+  A.fN[3] = ph.ne * ph.td / (d.lifetime * d.lifetime);
+  A.fS[0] = -ph.st * dv / (2 * v);
+  A.fS[1] = d.tran_diff > 0 ? ph.st / (2 * d.tran_diff) : 0.0;
+  A.fL[0] = -1.5 * ph.lam * dv / v;
+  A.fL[1] = d.long_diff > 0 ? ph.lam / (2 * d.long_diff) : 0.0;
+  A.fF = -ph.tau * dv / v;
+  const double s0 = d.sig[ph.idx - 1], s1 = d.sig[ph.idx], s2 = d.sig[ph.idx + 1];
+  const double l = ph.lam;
+  A.dl[0] = (2 * l - s1 - s2) / ((s0 - s1) * (s0 - s2));
+  A.dl[1] = (2 * l - s0 - s2) / ((s1 - s0) * (s1 - s2));
+  A.dl[2] = (2 * l - s0 - s1) / ((s2 - s0) * (s2 - s1));
+#pragma unroll
+  for (int k = 0; k < kMaxNb; ++k) {
+    A.dwx[k] = k < d.nb ? (float)dwx[k] : 0.f;
+    A.dwy[k] = k < d.nb ? (float)dwy[k] : 0.f;
+  }
+  A.pad[0] = A.pad[1] = 0.f;
+  adj[s] = A;
 }
 
-// sum_{o=lo..hi} G(o) * X[o - S + sh] using the window prefix sums CX (L+1 entries); pieces
-// before q0 end below lo.
-__device__ __forceinline__ double g_dot(const Pieces* __restrict__ pc, int n, int q0, const double* __restrict__ CX, int lo,
-                                        int hi, int S, int sh) {
-  double acc = 0.0;
-  int prev = q0 > 0 ? pc->x[q0 - 1] : -1;  // pieces cover (prev, x_q]
-  for (int q = q0; q < n && prev < hi; ++q) {
-    const int xq = pc->x[q];
-    const int a = (prev + 1) > lo ? (prev + 1) : lo;
-    const int b = xq < hi ? xq : hi;
-    if (a <= b) acc += (double)pc->c[q] * (CX[b - S + sh + 1] - CX[a - S + sh]);
+// The window dot products of one (segment, row) pair against the row's step function G held in
+// LDS (pieces px/pc, np of them): y1 = sum_{o=lo1..hi1} G(o) X[o-S], y0 = sum_{o=lo0..hi0} G(o)
+// X[o-S+1], from the window prefix sums CX (CX[k] = sum_{j<k} X[j]); NT tables at stride ts.
+template <int NT>
+__device__ __forceinline__ void g_dots(const int* __restrict__ px, const float* __restrict__ pc, int np, int q0,
+                                       const double* __restrict__ CX, int64_t ts, int S, int lo1, int hi1, int lo0,
+                                       int hi0, double* y1, double* y0) {
+#pragma unroll
+  for (int k = 0; k < NT; ++k) y1[k] = y0[k] = 0.0;
+  int prev = q0 > 0 ? px[q0 - 1] : -1;  // piece q covers (prev, x_q]
+  for (int q = q0; q < np && prev < hi1; ++q) {
+    const int xq = px[q];
+    const double c = (double)pc[q];
+    const int a1 = (prev + 1) > lo1 ? (prev + 1) : lo1, b1 = xq < hi1 ? xq : hi1;
+    const int a0 = (prev + 1) > lo0 ? (prev + 1) : lo0, b0 = xq < hi0 ? xq : hi0;
+    if (a1 <= b1) {
+#pragma unroll
+      for (int k = 0; k < NT; ++k) y1[k] += c * (CX[k * ts + b1 - S + 1] - CX[k * ts + a1 - S]);
+    }
+    if (a0 <= b0) {
+#pragma unroll
+      for (int k = 0; k < NT; ++k) y0[k] += c * (CX[k * ts + b0 - S + 2] - CX[k * ts + a0 - S + 1]);
+    }
     prev = xq;
   }
-  return acc;
 }
 
-__global__ __launch_bounds__(256) void k_seg_adjoint(Dev d, SegIn in, Tables tb, const uint32_t* __restrict__ keys,
-                                                     const uint32_t* __restrict__ vals, int64_t n,
-                                                     const int* __restrict__ cell_row, const Pieces* __restrict__ pieces,
-                                                     int64_t max_rows, double* __restrict__ partial) {
-  __shared__ double red[kNGrad][256 / kWave];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double gr[kNGrad];
+// Contribution of segment record R (chain factors A) to the 7 gradients through pixel row
+// (main pixel + (a, b)): the neighbour stripe unless (a, b) = (0, 0), and the diffusion bins
+// landing on the row when |a|, |b| <= 1.  Forward placement (sim_jax.py:170-261): (1-f) V at
+// o = S + j, f V at o = S - 1 + j, (1-f) Delta at S - 1 and f Delta at S - 2, S = nt - L - k0.
+template <int NB>
+__device__ __forceinline__ void pair_adjoint(const Dev& d, const Tables& tb, const SegRec& R,
+                                             const AdjRec* __restrict__ Ap, int a, int b,
+                                             const int* __restrict__ px, const float* __restrict__ pc, int np,
+                                             double* g) {
+  constexpr int SYM = (NB - 1) / 2;
+  const int L = d.L, nt = d.nt, nout = d.nout, nsb = d.nsb;
+  const int S = nt - L - R.k0;
+  int q0 = 0;
+  while (q0 < np && px[q0] < S - 2) ++q0;
+  if (q0 == np) return;  // G vanishes from tick S - 2 on
+  const double f = (double)R.f, ne = (double)R.ne;
+  // G(S - 2) = c[q0]; G(S - 1) is c[q0] unless piece q0 ends at S - 2
+  const double gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? (double)pc[q0] : 0.0;
+  double gc0 = 0.0;
+  if (S - 1 >= 0 && S - 1 <= nout - 2) {
+    const int q = px[q0] >= S - 1 ? q0 : q0 + 1;
+    gc0 = q < np ? (double)pc[q] : 0.0;
+  }
+  const double gc = (1.0 - f) * gc0 + f * gc1;
+  const int lo1 = S > 1 ? S : 1, hi1 = (S + L - 1) < (nout - 1) ? (S + L - 1) : (nout - 1);
+  const int lo0 = (S - 1) > 1 ? (S - 1) : 1, hi0 = (S + L - 2) < (nout - 1) ? (S + L - 2) : (nout - 1);
+  const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
+  double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
+  if (a != 0 || b != 0) {
+    const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
+    double y1, y0;
+    g_dots<1>(px, pc, np, q0, tb.CTN + sidx * (L + 1), 0, S, lo1, hi1, lo0, hi0, &y1, &y0);
+    const float2 D = tb.DN[sidx * nt + R.k0];
+    const double corr = (double)D.x - f * (double)D.y;
+    gN += (1.0 - f) * y1 + f * y0 + gc * corr;
+    gF += ne * (y0 - y1 + (gc1 - gc0) * corr - gc * (double)D.y);
+  }
+  bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
+  if (NB == 1 && !(a == 0 && b == 0)) near = false;
+  if (near) {
+    const int cm = d.cm;
+    const int64_t tstride = (int64_t)cm * cm * (L + 1);
+    const double la = (double)R.la, lb = (double)R.lb, lc = (double)R.lc;
+    const double da = Ap->dl[0], db = Ap->dl[1], dc = Ap->dl[2];
 #pragma unroll
-  for (int k = 0; k < kNGrad; ++k) gr[k] = 0.0;
-  if (i < n && keys[i] < (uint32_t)d.n_cells) {
-    Phys ph;
-    seg_physics(d, in, vals[i], ph);
-    if (ph.ne != 0.0) {
-      double wx[kMaxNb], wy[kMaxNb], dwx[kMaxNb], dwy[kMaxNb];
-      tran_weights(d, ph.x0, ph.st, wx, dwx);
-      tran_weights(d, ph.y0, ph.st, wy, dwy);
-      const double s0 = d.sig[ph.idx - 1], s1 = d.sig[ph.idx], s2 = d.sig[ph.idx + 1];
-      const double l = ph.lam;
-      const double da = (2 * l - s1 - s2) / ((s0 - s1) * (s0 - s2));
-      const double db = (2 * l - s0 - s2) / ((s1 - s0) * (s1 - s2));
-      const double dc = (2 * l - s0 - s1) / ((s2 - s0) * (s2 - s1));
-      // forward used float32 copies of these; the adjoint linearises the same quantities
-      const double ne = (double)(float)ph.ne;
-      const double f = (double)(float)ph.f;
-      const double la = (double)(float)ph.la, lb = (double)(float)ph.lb, lc = (double)(float)ph.lc;
-      const int L = d.L, nt = d.nt, nsb = d.nsb, cm = d.cm;
-      const int S = nt - L - ph.k0;
-      const int ev = ph.ev, plane = ph.plane;
-      const int ux = pymod_i(ph.bx, nsb), uy = pymod_i(ph.by, nsb);
-      const int64_t tstride = (int64_t)cm * cm * (L + 1);
-      double gN = 0.0, gS = 0.0, gF = 0.0, gL = 0.0;
-      const int nout = d.nout;
-      const int lo1 = S > 1 ? S : 1;                                   // (1-f) window: o = S + j
-      const int hi1 = (S + L - 1) < (nout - 1) ? (S + L - 1) : (nout - 1);
-      const int lo0 = (S - 1) > 1 ? (S - 1) : 1;                       // f window: o = S - 1 + j
-      const int hi0 = (S + L - 2) < (nout - 1) ? (S + L - 2) : (nout - 1);
-      for (int a = -d.nn; a <= d.nn; ++a) {
-        const int tx = ph.mpx + a;
-        if (tx < 0 || tx >= d.nx) continue;
-        for (int b = -d.nn; b <= d.nn; ++b) {
-          const int ty = ph.mpy + b;
-          if (ty < 0 || ty >= d.ny) continue;
-          const int64_t c = cell_of(d, ev, plane, tx, ty);
-          const int r = cell_row[c];
-          if (r < 0 || r >= max_rows) continue;
-          const bool has_nb = (a != 0) || (b != 0);
-          unsigned imask = 0, jmask = 0;
-          if (a >= -1 && a <= 1 && b >= -1 && b <= 1) {
-            for (int q = 0; q < d.nb; ++q) {
-              if (floordiv_i(ph.bx + q - d.nbsym, nsb) == tx) imask |= 1u << q;
-              if (floordiv_i(ph.by + q - d.nbsym, nsb) == ty) jmask |= 1u << q;
-            }
-          }
-          const bool has_bins = imask && jmask;
-          if (!has_nb && !has_bins) continue;
-          const Pieces* pc = pieces + r;
-          const int np = pc->n;
-          if (np == 0) continue;
-          // first piece reaching the earliest tick this segment touches (S - 2)
-          int q0 = 0;
-          while (q0 < np && pc->x[q0] < S - 2) ++q0;
-          if (q0 == np) continue;
-          // correction cotangents: (1-f) Delta at o = S-1, f Delta at o = S-2 (valid for o >= 0)
-          const double gc0 = (S - 1 >= 0 && S - 1 <= nout - 2) ? g_at(pc, np, q0, S - 1) : 0.0;
-          const double gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? g_at(pc, np, q0, S - 2) : 0.0;
-          const double gc = (1.0 - f) * gc0 + f * gc1;
-          if (has_nb) {
-            const int cxN = (int)fabsf((float)(ux - nsb / 2) + 0.5f - (float)(a * nsb));
-            const int cyN = (int)fabsf((float)(uy - nsb / 2) + 0.5f - (float)(b * nsb));
-            const int64_t sidx = (int64_t)cxN * d.ly + cyN;
-            const double* CX = tb.CTN + sidx * (L + 1);
-            const double y1 = g_dot(pc, np, q0, CX, lo1, hi1, S, 0);
-            const double y0 = g_dot(pc, np, q0, CX, lo0, hi0, S, 1);
-            const float2 D = tb.DN[sidx * nt + ph.k0];
-            const double corr = (double)D.x - f * (double)D.y;
-            gN += (1.0 - f) * y1 + f * y0 + gc * corr;
-            gF += ne * (y0 - y1 + (gc1 - gc0) * corr - gc * (double)D.y);
-          }
-          if (has_bins) {
-            for (int q = 0; q < d.nb; ++q) {
-              if (!((imask >> q) & 1)) continue;
-              const int cx = (int)fabsf((float)(pymod_i(ph.bx + q - d.nbsym, nsb) - nsb / 2) + 0.5f);
-              for (int qq = 0; qq < d.nb; ++qq) {
-                if (!((jmask >> qq) & 1)) continue;
-                const int cy = (int)fabsf((float)(pymod_i(ph.by + qq - d.nbsym, nsb) - nsb / 2) + 0.5f);
-                const int64_t base = (((int64_t)(ph.idx - 1) * cm + cx) * cm + cy) * (L + 1);
-                const double* C1 = tb.CTM + base;
-                const double* C2 = C1 + tstride;
-                const double* C3 = C2 + tstride;
-                const double y1a = g_dot(pc, np, q0, C1, lo1, hi1, S, 0), y0a = g_dot(pc, np, q0, C1, lo0, hi0, S, 1);
-                const double y1b = g_dot(pc, np, q0, C2, lo1, hi1, S, 0), y0b = g_dot(pc, np, q0, C2, lo0, hi0, S, 1);
-                const double y1c = g_dot(pc, np, q0, C3, lo1, hi1, S, 0), y0c = g_dot(pc, np, q0, C3, lo0, hi0, S, 1);
-                const double y1 = lb * y1b + la * y1a + lc * y1c;
-                const double y0 = lb * y0b + la * y0a + lc * y0c;
-                const float2 D = tb.DM[(((int64_t)ph.idx * cm + cx) * cm + cy) * nt + ph.k0];
-                const double corr = (double)D.x - f * (double)D.y;
-                const double wxy = (double)(float)wx[q] * (double)(float)wy[qq];
-                const double nbin = ne * wxy;
-                const double gn = (1.0 - f) * y1 + f * y0 + gc * corr;  // d loss / d n_bin
-                gN += gn * wxy;
-                gS += gn * ne * (dwx[q] * wy[qq] + wx[q] * dwy[qq]);
-                gF += nbin * (y0 - y1 + (gc1 - gc0) * corr - gc * (double)D.y);
-                gL += nbin * ((1.0 - f) * (da * y1a + db * y1b + dc * y1c) + f * (da * y0a + db * y0b + dc * y0c));
-              }
-            }
-          }
-        }
+    for (int i = 0; i < NB; ++i) {
+      const int vx = ux + i - SYM;
+      const int dxp = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
+      if (dxp != a) continue;
+      const int cx = offset_bin(vx - dxp * nsb, 0, nsb);
+      const double wxi = (double)R.wx[i], dwxi = (double)Ap->dwx[i];
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        const int vy = uy + jj - SYM;
+        const int dyp = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
+        if (dyp != b) continue;
+        const int cy = offset_bin(vy - dyp * nsb, 0, nsb);
+        const double wyj = (double)R.wy[jj];
+        const double* C1 = tb.CTM + (((int64_t)(R.idx - 1) * cm + cx) * cm + cy) * (L + 1);
+        double y1[3], y0[3];
+        g_dots<3>(px, pc, np, q0, C1, tstride, S, lo1, hi1, lo0, hi0, y1, y0);
+        const float2 D = tb.DM[(((int64_t)R.idx * cm + cx) * cm + cy) * nt + R.k0];
+        const double corr = (double)D.x - f * (double)D.y;
+        const double wxy = wxi * wyj;
+        const double Y1 = la * y1[0] + lb * y1[1] + lc * y1[2];
+        const double Y0 = la * y0[0] + lb * y0[1] + lc * y0[2];
+        const double gn = (1.0 - f) * Y1 + f * Y0 + gc * corr;  // d loss / d n_bin
+        gN += gn * wxy;
+        gS += gn * ne * (dwxi * wyj + wxi * (double)Ap->dwy[jj]);
+        gF += ne * wxy * (Y0 - Y1 + (gc1 - gc0) * corr - gc * (double)D.y);
+        gL += ne * wxy * ((1.0 - f) * (da * y1[0] + db * y1[1] + dc * y1[2]) +
+                          f * (da * y0[0] + db * y0[1] + dc * y0[2]));
       }
-      // chain rule to the parameters
-      const double v = d.v, dv = d.dv;
-      const double att = exp(-ph.td / d.lifetime) * (double)ph.mask;
-      const double dne_dAb = ph.dq_dAb * att;
-      const double dne_dkb = ph.dq_dkb * att;
-      const double dne_dE = ph.dq_dE_rec * att + ph.ne * ph.td * dv / (v * d.lifetime);
-      const double dne_dtau = ph.ne * ph.td / (d.lifetime * d.lifetime);
-      const double dst_dDT = d.tran_diff > 0 ? ph.st / (2 * d.tran_diff) : 0.0;
-      const double dst_dE = -ph.st * dv / (2 * v);
-      const double dlam_dDL = d.long_diff > 0 ? ph.lam / (2 * d.long_diff) : 0.0;
-      const double dlam_dE = -1.5 * ph.lam * dv / v;
-      const double df_dE = -ph.tau * dv / v;
-      gr[0] = gN * dne_dAb;
-      gr[1] = gN * dne_dkb;
-      gr[2] = gN * dne_dE + gS * dst_dE + gF * df_dE + gL * dlam_dE;
-      gr[3] = gN * dne_dtau;
-      gr[4] = 0.0;  // vdrift is never read by the simulation (drift speed comes from eField)
-      gr[5] = gL * dlam_dDL;
-      gr[6] = gS * dst_dDT;
     }
   }
-  // fixed-order block reduction
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-#pragma unroll
-  for (int k = 0; k < kNGrad; ++k) {
-    const double s = wave_sum(gr[k]);
-    if (lane == 0) red[k][wid] = s;
+  g[0] += gN * Ap->fN[0];
+  g[1] += gN * Ap->fN[1];
+  g[2] += gN * Ap->fN[2] + gF * Ap->fF;
+  g[3] += gN * Ap->fN[3];
+  if (near) {
+    g[2] += gS * Ap->fS[0] + gL * Ap->fL[0];
+    g[5] += gL * Ap->fL[1];
+    g[6] += gS * Ap->fS[1];
+  }
+}
+
+// Stage 2, row-owned: one workgroup per pixel row holds the row's step function in LDS and walks
+// the segments of the (2n+1)^2 neighbourhood cells (the same pairs the forward's waveform kernel
+// visits), every thread taking pairs from the flattened list; fixed-order block reduction to the
+// row's 7 partial gradients.  Rows without gradient (no valid hit) exit at once.
+template <int NB, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, const SegRec* __restrict__ rec,
+                                                            const AdjRec* __restrict__ adj, const int2* __restrict__ rng,
+                                                            const int* __restrict__ row_cell,
+                                                            const int* __restrict__ n_rows_dev, int64_t max_rows,
+                                                            const Pieces* __restrict__ pieces,
+                                                            double* __restrict__ partial) {
+  constexpr int NT = WAVES * kWave;
+  __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
+  __shared__ int n_cells_tot;
+  __shared__ int spx[2 * kMaxH];
+  __shared__ float spc[2 * kMaxH];
+  __shared__ double red[kNGrad][NT / kWave];
+  const int t = threadIdx.x;
+  const int P0 = *n_rows_dev;
+  const int P = P0 < max_rows ? P0 : (int)max_rows;
+  if ((int64_t)blockIdx.x >= P) {
+    if (blockIdx.x < max_rows && t < kNGrad) partial[(int64_t)blockIdx.x * kNGrad + t] = 0.0;
+    return;
+  }
+  // rows in XCD bands, as in the forward
+  const int xcd = (int)(blockIdx.x % kXcd);
+  const int q8 = P / kXcd, rem = P % kXcd;
+  const int r = xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
+  const Pieces* pr = pieces + r;
+  const int np = pr->n;
+  if (np == 0) {
+    if (t < kNGrad) partial[(int64_t)r * kNGrad + t] = 0.0;
+    return;
+  }
+  if (t < np) {
+    spx[t] = pr->x[t];
+    spc[t] = pr->c[t];
+  }
+  if (t == 0) {
+    const int cell = row_cell[r];
+    const int px = cell % d.gx - d.halo;
+    const int py = (cell / d.gx) % d.gy - d.halo;
+    const int evp = cell / (d.gx * d.gy);
+    int k = 0, cum = 0;
+    c_cum[0] = 0;
+    for (int pass = 0; pass < 2; ++pass) {  // neighbour-only cells first, then the 3x3 near cells
+      for (int a = -d.nn; a <= d.nn; ++a) {
+        for (int b = -d.nn; b <= d.nn; ++b) {
+          const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
+          if ((int)near != pass) continue;
+          const int qx = px - a, qy = py - b;
+          if (qx < -d.halo || qx >= d.nx + d.halo || qy < -d.halo || qy >= d.ny + d.halo) continue;
+          const int2 rg = rng[((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo)];
+          if (rg.y <= rg.x) continue;
+          c_beg[k] = rg.x;
+          cum += rg.y - rg.x;
+          c_cum[k + 1] = cum;
+          c_ab[k] = ((a + 64) << 8) | (b + 64);
+          ++k;
+        }
+      }
+    }
+    n_cells_tot = k;
   }
   __syncthreads();
-  if (threadIdx.x < kNGrad) {
-    double s = 0.0;
-    for (int w = 0; w < 256 / kWave; ++w) s += red[threadIdx.x][w];
-    partial[(int64_t)blockIdx.x * kNGrad + threadIdx.x] = s;
+  double g[kNGrad];
+#pragma unroll
+  for (int k = 0; k < kNGrad; ++k) g[k] = 0.0;
+  const int nct = n_cells_tot;
+  const int tot = c_cum[nct];
+  int cur = 0;
+  for (int i = t; i < tot; i += NT) {
+    while (i >= c_cum[cur + 1]) ++cur;
+    const int s = c_beg[cur] + (i - c_cum[cur]);
+    const int ab = c_ab[cur];
+    const SegRec R = rec[s];
+    if (R.ne == 0.f) continue;
+    pair_adjoint<NB>(d, tb, R, adj + s, (ab >> 8) - 64, (ab & 255) - 64, spx, spc, np, g);
+  }
+  const int lane = t & (kWave - 1), wid = t / kWave;
+#pragma unroll
+  for (int k = 0; k < kNGrad; ++k) {
+    const double v = wave_sum(g[k]);
+    if (lane == 0) red[k][wid] = v;
+  }
+  __syncthreads();
+  if (t < kNGrad) {
+    double v = 0.0;
+    for (int w = 0; w < NT / kWave; ++w) v += red[t][w];
+    partial[(int64_t)r * kNGrad + t] = v;
   }
 }
 

@@ -599,7 +599,8 @@ int lsim_backward_query(lsim_ctx* c, int64_t n_seg, int64_t max_rows, size_t* by
   Carve cv(nullptr);
   cv.take<Pieces>(max_rows);
   cv.take<double>(max_rows);
-  cv.take<double>(((n_seg + 255) / 256) * kNGrad);
+  cv.take<double>(max_rows * kNGrad);
+  cv.take<AdjRec>(n_seg);
   *bytes = cv.off + 256;
   return LSIM_OK;
 }
@@ -626,19 +627,29 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
   Carve cv(bwd_ws);
   Pieces* pieces = cv.take<Pieces>(max_rows);
   double* dz = cv.take<double>(max_rows);
-  const int64_t nblk = (n + 255) / 256;
-  double* partial = cv.take<double>(nblk * kNGrad);
+  double* partial = cv.take<double>(max_rows * kNGrad);
+  AdjRec* adj = cv.take<AdjRec>(n);
   Tables tb{c->TM3, c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
   SegIn in = make_segin(seg);
   uint32_t *keys, *vals;
   sorted_buffers(w, d.n_cells, keys, vals);
-  if (max_rows > 0)
+  if (max_rows > 0) {
     hipLaunchKernelGGL(k_fee_adjoint, dim3((unsigned)((max_rows + 63) / 64)), dim3(64), 0, st, d, (const FeeTape*)fwd->tape,
                        d_adc, d_z, fwd->n_rows, max_rows, pieces, dz);
-  if (nblk > 0)
-    hipLaunchKernelGGL(k_seg_adjoint, dim3((unsigned)nblk), dim3(256), 0, st, d, in, tb, keys, vals, n, w.cell_row, pieces,
-                       max_rows, partial);
-  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(256), 0, st, d, partial, nblk, dz, fwd->n_rows, max_rows, d_params);
+    if (n > 0)
+      hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, in, keys, vals, n, adj);
+    const dim3 grid((unsigned)max_rows), block(kAdjWaves * kWave);
+    if (d.nb == 5)
+      hipLaunchKernelGGL((k_rows_adj<5, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell, w.n_rows,
+                         max_rows, pieces, partial);
+    else if (d.nb == 3)
+      hipLaunchKernelGGL((k_rows_adj<3, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell, w.n_rows,
+                         max_rows, pieces, partial);
+    else
+      hipLaunchKernelGGL((k_rows_adj<1, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell, w.n_rows,
+                         max_rows, pieces, partial);
+  }
+  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(256), 0, st, d, partial, max_rows, dz, fwd->n_rows, max_rows, d_params);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }
