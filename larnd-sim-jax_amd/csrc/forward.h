@@ -116,10 +116,16 @@ __global__ void k_copy_count(const int* __restrict__ src, int64_t* __restrict__ 
 // 3. the waveform kernel: one workgroup owns one pixel row
 // ------------------------------------------------------------------------------------------
 
+// Window tables used by the waveform kernels are zero padded: P[p] = W[p - 2] for 2 <= p < L + 2
+// and 0 elsewhere, kWinPad entries per series, so chunk c of lane l reads P[63 c + l] with no
+// bounds test (windows up to 3 * 63 - 2 = 187 samples).
+constexpr int kWinPad = 3 * kWave;
+
 struct Tables {
-  const float3* TM3;  // [nm][cm][cm][L]   (T_{m-1}, T_m, T_{m+1}) windows for the diffusion bins
+  const float3* TM3;  // [nm][cm][cm][kWinPad] padded (T_{m-1}, T_m, T_{m+1}) windows of the diffusion bins
   const float* TM;    // [nm][cm][cm][L]   main-bin template windows
   const float* TN;    // [lx][ly][L]       raw response window (neighbours)
+  const float* TNP;   // [lx][ly][kWinPad] the same, padded
   const float2* DM;   // [nm][cm][cm][nt]  (C[nt-L]-C[k], C[min(k+1,nt-1)]-C[k]) of template m
   const float2* DN;   // [lx][ly][nt]      same for the raw response
   const double* CTM;  // [nm][cm][cm][L+1] window prefix sums (adjoint)
@@ -366,12 +372,12 @@ __device__ __forceinline__ void far_load(const Dev& d, const Tables& tb, const F
   const int cxN = offset_bin(R.uxy & 0xffff, a, d.nsb);
   const int cyN = offset_bin(R.uxy >> 16, b, d.nsb);
   const int sidx = cxN * d.ly + cyN;
-  const float* TNp = tb.TN + sidx * L;
+  const float* TNp = tb.TNP + sidx * kWinPad + lane;
   q.D = tb.DN[sidx * d.nt + R.k0];
-  const int m0 = lane - 2;
-  q.t0 = (m0 >= 0) ? TNp[m0] : 0.f;
-  q.t1 = (m0 + kChunk < L) ? TNp[m0 + kChunk] : 0.f;
-  q.t2 = (m0 + 2 * kChunk < L) ? TNp[m0 + 2 * kChunk] : 0.f;
+  q.t0 = TNp[0];
+  q.t1 = TNp[kChunk];
+  q.t2 = TNp[2 * kChunk];
+  (void)L;
 }
 
 __device__ __forceinline__ void far_store(const Dev& d, const FarRec& R, const FarLoads& q, float* accw, int lane) {
@@ -413,8 +419,6 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
   const int idx = R.idx, k0 = R.k0;
   const float f = R.f, omf = 1.f - f;
   const float la = R.la, lb = R.lb, lc = R.lc;
-  const int m0 = lane - 2;
-  const bool ok0 = m0 >= 0, ok1 = m0 + kChunk < L, ok2 = m0 + 2 * kChunk < L;
   float delta = 0.f;
   float v0 = 0.f, v1 = 0.f, v2 = 0.f;
   if (has_nb) {
@@ -423,15 +427,14 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
     const int64_t sidx = (int64_t)cxN * d.ly + cyN;
     const float2 D = tb.DN[sidx * d.nt + k0];
     delta = R.ne * (D.x - f * D.y);
-    const float* TNp = tb.TN + sidx * L;
+    const float* TNp = tb.TNP + sidx * kWinPad + lane;
     const float ne = R.ne;
-    if (ok0) v0 = ne * TNp[m0];
-    if (ok1) v1 = ne * TNp[m0 + kChunk];
-    if (ok2) v2 = ne * TNp[m0 + 2 * kChunk];
+    v0 = ne * TNp[0];
+    v1 = ne * TNp[kChunk];
+    v2 = ne * TNp[2 * kChunk];
   }
   if (has_bins) {
-    const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * L;
-    const float3 z3 = make_float3(0.f, 0.f, 0.f);
+    const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * kWinPad + lane;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       // one uniform branch per row of bins: the row's 3 * NB loads overlap, rows do not (bounds
@@ -445,10 +448,10 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
             const float2 D = tb.DM[((int64_t)idx * cm * cm + coff) * d.nt + k0];
             delta += c * (D.x - f * D.y);
           }
-          const float3* tp = T3 + coff * L + m0;
-          const float3 q0 = ok0 ? tp[0] : z3;
-          const float3 q1 = ok1 ? tp[kChunk] : z3;
-          const float3 q2 = ok2 ? tp[2 * kChunk] : z3;
+          const float3* tp = T3 + coff * kWinPad;
+          const float3 q0 = tp[0];
+          const float3 q1 = tp[kChunk];
+          const float3 q2 = tp[2 * kChunk];
           v0 += c * (q0.y * lb + q0.x * la + q0.z * lc);
           v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
           v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
@@ -457,26 +460,6 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
     }
   }
   store_window(accw, lane, v0, v1, v2, 1.f, omf, f, delta, d.nt - L - k0, L, d.nout);
-}
-
-// Any window length (L > 187): one chunk at a time, every bound checked.
-__device__ __forceinline__ void far_pair_any(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, float* accw,
-                                             int lane) {
-  const int nsb = d.nsb, L = d.L;
-  const int cxN = offset_bin(R.uxy & 0xffff, a, nsb);
-  const int cyN = offset_bin(R.uxy >> 16, b, nsb);
-  const int64_t sidx = (int64_t)cxN * d.ly + cyN;
-  const float* TNp = tb.TN + sidx * L;
-  const float2 D = tb.DN[sidx * d.nt + R.k0];
-  const float ne = R.ne, f = R.f, omf = 1.f - f;
-  const float delta = ne * (D.x - f * D.y);
-  const int S = d.nt - L - R.k0;
-  for (int base = -2; base < L; base += kChunk) {
-    const int m = base + lane;
-    const float v = (m >= 0 && m < L) ? ne * TNp[m] : 0.f;
-    const float vn = __shfl_down(v, 1, kWave);
-    scatter_chunk(accw, lane, m, v, vn, omf, f, delta, S, L, d.nout);
-  }
 }
 
 // Segment-parallel window of the diffusion bins that land on the segment's own (main) pixel:
@@ -508,51 +491,45 @@ __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* 
   }
   const float f = R.f, la = R.la, lb = R.lb, lc = R.lc;
   const int idx = R.idx, k0 = R.k0;
-  const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * L;
+  const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * kWinPad + lane;
   float delta = 0.f;
-  for (int j0 = 0; j0 < L; j0 += 3 * kWave) {
-    const int j1 = j0 + lane, j2 = j1 + kWave, j3 = j2 + kWave;
-    const bool ok1 = j1 < L, ok2 = j2 < L, ok3 = j3 < L;
-    const float3 z3 = make_float3(0.f, 0.f, 0.f);
-    float v1 = 0.f, v2 = 0.f, v3 = 0.f;
+  // padded positions p = lane, lane + 64, lane + 128 (kWinPad = 3 * 64)
+  float v1 = 0.f, v2 = 0.f, v3 = 0.f;
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      if (wxs[i] != 0.f) {
+  for (int i = 0; i < NB; ++i) {
+    if (wxs[i] != 0.f) {
 #pragma unroll
-        for (int jj = 0; jj < NB; ++jj) {
-          const float c = wxs[i] * wys[jj];
-          const int coff = cxs[i] * cm + cys[jj];
-          if (j0 == 0 && c != 0.f) {
-            const float2 D = tb.DM[((int64_t)idx * cm * cm + coff) * d.nt + k0];
-            delta += c * (D.x - f * D.y);
-          }
-          const float3* tp = T3 + (int64_t)coff * L;
-          const float3 q1 = ok1 ? tp[j1] : z3;
-          const float3 q2 = ok2 ? tp[j2] : z3;
-          const float3 q3 = ok3 ? tp[j3] : z3;
-          v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
-          v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
-          v3 += c * (q3.y * lb + q3.x * la + q3.z * lc);
+      for (int jj = 0; jj < NB; ++jj) {
+        const float c = wxs[i] * wys[jj];
+        const int coff = cxs[i] * cm + cys[jj];
+        if (c != 0.f) {
+          const float2 D = tb.DM[((int64_t)idx * cm * cm + coff) * d.nt + k0];
+          delta += c * (D.x - f * D.y);
         }
+        const float3* tp = T3 + (int64_t)coff * kWinPad;
+        const float3 q1 = tp[0], q2 = tp[kWave], q3 = tp[2 * kWave];
+        v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
+        v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
+        v3 += c * (q3.y * lb + q3.x * la + q3.z * lc);
       }
     }
-    float* o = vc + (int64_t)s * L;
-    if (ok1) o[j1] = v1;
-    if (ok2) o[j2] = v2;
-    if (ok3) o[j3] = v3;
   }
+  float* o = vc + (int64_t)s * kWinPad + lane;
+  o[0] = v1;
+  o[kWave] = v2;
+  o[2 * kWave] = v3;
+  (void)L;
   if (lane == 0) dc[s] = delta;
 }
 
 // Center pair from the precomputed window: V = vc[s] (charge included), Delta = dc[s].
 __device__ __forceinline__ void center_load(const Dev& d, const float* __restrict__ vc, const float* __restrict__ dc,
                                             int s, int lane, FarLoads& q) {
-  const int L = d.L;
-  const float* V = vc + (int64_t)s * L;
-  const int m0 = lane - 2;
-  q.t0 = (m0 >= 0) ? V[m0] : 0.f;
-  q.t1 = (m0 + kChunk < L) ? V[m0 + kChunk] : 0.f;
-  q.t2 = (m0 + 2 * kChunk < L) ? V[m0 + 2 * kChunk] : 0.f;
+  const float* V = vc + (int64_t)s * kWinPad + lane;
+  q.t0 = V[0];
+  q.t1 = V[kChunk];
+  q.t2 = V[2 * kChunk];
+  (void)d;
   q.D = make_float2(dc[s], 0.f);
 }
 
@@ -641,42 +618,28 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
 
   float* accw = acc + wid * rstride;
   const int nfc = n_far_cells, nct = n_cells_tot;
-  const int far_total = c_cum[nfc], tot = c_cum[nct];
+  const int tot = c_cum[nct];
   // far pairs: cell by cell, kFarGroup independent segments per step (wave w takes the cell's
-  // segments w, w + WAVES, ...); then the near pairs
-  if (d.L <= 3 * kChunk - 2) {
-    for (int c = 0; c < nfc; ++c) {
-      const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
-      const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
-      const int ab = __builtin_amdgcn_readfirstlane(c_ab[c]);
-      const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
-      const FarRec* fr = reinterpret_cast<const FarRec*>(rec + beg);
-      for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
-        FarRec R[kFarGroup];
+  // segments w, w + WAVES, ...); then the near pairs (windows of at most 187 samples)
+  for (int c = 0; c < nfc; ++c) {
+    const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
+    const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
+    const int ab = __builtin_amdgcn_readfirstlane(c_ab[c]);
+    const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+    for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
+      FarRec R[kFarGroup];
 #pragma unroll
-        for (int g = 0; g < kFarGroup; ++g) {
-          const int j = k + g * WAVES;
-          R[g] = *reinterpret_cast<const FarRec*>(rec + beg + (j < cnt ? j : k));
-          if (j >= cnt) R[g].ne = 0.f;
-        }
-        FarLoads q[kFarGroup];
-#pragma unroll
-        for (int g = 0; g < kFarGroup; ++g) far_load(d, tb, R[g], a, b, lane, q[g]);
-#pragma unroll
-        for (int g = 0; g < kFarGroup; ++g)
-          if (R[g].ne != 0.f) far_store(d, R[g], q[g], accw, lane);
+      for (int g = 0; g < kFarGroup; ++g) {
+        const int j = k + g * WAVES;
+        R[g] = *reinterpret_cast<const FarRec*>(rec + beg + (j < cnt ? j : k));
+        if (j >= cnt) R[g].ne = 0.f;
       }
-      (void)fr;
-    }
-  } else {
-    int cur = 0;
-    for (int i = wid; i < far_total; i += WAVES) {
-      while (i >= c_cum[cur + 1]) ++cur;
-      const int s = __builtin_amdgcn_readfirstlane(c_beg[cur] + (i - c_cum[cur]));
-      const int ab = __builtin_amdgcn_readfirstlane(c_ab[cur]);
-      const SegRec R = rec[s];
-      if (R.ne == 0.f) continue;
-      far_pair_any(d, tb, R, (ab >> 8) - 64, (ab & 255) - 64, accw, lane);
+      FarLoads q[kFarGroup];
+#pragma unroll
+      for (int g = 0; g < kFarGroup; ++g) far_load(d, tb, R[g], a, b, lane, q[g]);
+#pragma unroll
+      for (int g = 0; g < kFarGroup; ++g)
+        if (R[g].ne != 0.f) far_store(d, R[g], q[g], accw, lane);
     }
   }
   // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
@@ -801,7 +764,8 @@ __global__ void k_hits(Dev d, const int64_t* __restrict__ pids, const float* __r
 // main != 0: m in [0, nm), (cx, cy) in [0, cm)^2 -> TM, DM, CTM; else the raw response over all
 // (cx, cy) -> TN, DN, CTN.
 __global__ __launch_bounds__(256) void k_lut_tables(Dev d, const float* __restrict__ raw, int main_tables, float* __restrict__ Tw,
-                                                    float2* __restrict__ Dt, double* __restrict__ CT) {
+                                                    float* __restrict__ Twp, float2* __restrict__ Dt,
+                                                    double* __restrict__ CT) {
   __shared__ double T[kMaxOut];
   __shared__ double g[128];
   __shared__ double shd[4];
@@ -869,6 +833,9 @@ __global__ __launch_bounds__(256) void k_lut_tables(Dev d, const float* __restri
   __syncthreads();
   const int64_t sbase = (int64_t)series;
   for (int j = threadIdx.x; j < L; j += blockDim.x) Tw[sbase * L + j] = win[j];
+  if (Twp)
+    for (int p = threadIdx.x; p < kWinPad; p += blockDim.x)
+      Twp[sbase * kWinPad + p] = (p >= 2 && p < L + 2) ? win[p - 2] : 0.f;
   const double cref = T[nt - L];
   for (int k = threadIdx.x; k < nt; k += blockDim.x) {
     const double c0 = T[k];
@@ -885,14 +852,20 @@ __global__ __launch_bounds__(256) void k_lut_tables(Dev d, const float* __restri
   }
 }
 
-// (T_{m-1}, T_m, T_{m+1}) windows for m in [1, nm-2]; other slabs are zero.
+// Padded (T_{m-1}, T_m, T_{m+1}) windows for m in [1, nm-2]; other slabs are zero.
 __global__ void k_lut_tm3(Dev d, const float* __restrict__ TM, float3* __restrict__ TM3) {
-  const int64_t per = (int64_t)d.cm * d.cm * d.L;
-  const int64_t n = per * d.nm;
+  const int64_t cc = (int64_t)d.cm * d.cm;
+  const int64_t n = cc * d.nm * kWinPad;
+  const int64_t per = cc * d.L;  // one template slab of TM
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m = i / per;
+    const int p = (int)(i % kWinPad);
+    const int64_t series = i / kWinPad;  // m * cm * cm + cx * cm + cy
+    const int64_t m = series / cc;
     float3 v = make_float3(0.f, 0.f, 0.f);
-    if (m >= 1 && m <= d.nm - 2) v = make_float3(TM[i - per], TM[i], TM[i + per]);
+    if (m >= 1 && m <= d.nm - 2 && p >= 2 && p < d.L + 2) {
+      const int64_t t = series * d.L + (p - 2);
+      v = make_float3(TM[t - per], TM[t], TM[t + per]);
+    }
     TM3[i] = v;
   }
 }

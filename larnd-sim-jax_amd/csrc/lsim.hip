@@ -54,6 +54,7 @@ struct lsim_ctx {
   float3* TM3 = nullptr;
   float* TM = nullptr;
   float* TN = nullptr;
+  float* TNP = nullptr;
   float2* DM = nullptr;
   float2* DN = nullptr;
   double* CTM = nullptr;
@@ -203,11 +204,11 @@ Ws layout(void* base, int64_t n, int64_t n_cells) {
   return w;
 }
 
-// The centre windows need L floats per segment; carved after the rest.
-void layout_center(Ws& w, void* base, int64_t n, int L) {
+// The centre windows need kWinPad floats per segment (padded); carved after the rest.
+void layout_center(Ws& w, void* base, int64_t n) {
   Carve c(base);
   c.off = w.bytes;
-  w.vc = c.take<float>(n * L);
+  w.vc = c.take<float>(n * kWinPad);
   w.dc = c.take<float>(n);
   w.bytes = c.off + 256;
 }
@@ -268,11 +269,12 @@ void free_tables(lsim_ctx* c) {
   (void)hipFree(c->TM3);
   (void)hipFree(c->TM);
   (void)hipFree(c->TN);
+  (void)hipFree(c->TNP);
   (void)hipFree(c->DM);
   (void)hipFree(c->DN);
   (void)hipFree(c->CTM);
   (void)hipFree(c->CTN);
-  c->TM = c->TN = nullptr;
+  c->TM = c->TN = c->TNP = nullptr;
   c->TM3 = nullptr;
   c->DM = c->DN = nullptr;
   c->CTM = c->CTN = nullptr;
@@ -326,7 +328,8 @@ int lsim_ctx_create(lsim_ctx** out, const lsim_geometry* geo, const lsim_config*
   if (nsb < 2 || (nsb & 1)) return fail(LSIM_E_ARG, "nb_sampling_bins_per_pixel must be even and >= 2");
   if ((nb - 1) / 2 >= nsb) return fail(LSIM_E_ARG, "diffusion bins wider than a pixel are not supported");
   if (cfg->number_pix_neighbors < 0 || cfg->number_pix_neighbors > 8) return fail(LSIM_E_ARG, "number_pix_neighbors out of range");
-  if (cfg->signal_length < 1 || cfg->signal_length > 1024) return fail(LSIM_E_ARG, "signal_length out of range");
+  if (cfg->signal_length < 1 || cfg->signal_length > 3 * kChunk - 2)
+    return fail(LSIM_E_ARG, "signal_length out of range (1..187)");
   if (cfg->n_ticks < 4 || cfg->n_ticks - 1 > kMaxOut) return fail(LSIM_E_ARG, "n_ticks out of range (<= 2049)");
   if (cfg->max_adc_values < 1 || cfg->max_adc_values > kMaxH) return fail(LSIM_E_ARG, "max_adc_values out of range");
   if (cfg->n_templates < 3 || cfg->n_templates > LSIM_MAX_TEMPLATES) return fail(LSIM_E_ARG, "n_templates out of range");
@@ -366,9 +369,10 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
   const int64_t ms = (int64_t)k.n_templates * c->cm * c->cm, ns = (int64_t)nx * ny;
   const int L = k.signal_length;
   if (!c->has_lut) {
-    HIP_TRY(hipMalloc(&c->TM3, ms * L * sizeof(float3)));
+    HIP_TRY(hipMalloc(&c->TM3, ms * kWinPad * sizeof(float3)));
     HIP_TRY(hipMalloc(&c->TM, ms * L * sizeof(float)));
     HIP_TRY(hipMalloc(&c->TN, ns * L * sizeof(float)));
+    HIP_TRY(hipMalloc(&c->TNP, ns * kWinPad * sizeof(float)));
     HIP_TRY(hipMalloc(&c->DM, ms * nt * sizeof(float2)));
     HIP_TRY(hipMalloc(&c->DN, ns * nt * sizeof(float2)));
     HIP_TRY(hipMalloc(&c->CTM, ms * (L + 1) * sizeof(double)));
@@ -377,8 +381,8 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
   Dev d;
   int rc = make_dev(c, nullptr, 1, d);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ms), dim3(256), 0, st, d, raw, 1, c->TM, c->DM, c->CTM);
-  hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ns), dim3(256), 0, st, d, raw, 0, c->TN, c->DN, c->CTN);
+  hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ms), dim3(256), 0, st, d, raw, 1, c->TM, (float*)nullptr, c->DM, c->CTM);
+  hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ns), dim3(256), 0, st, d, raw, 0, c->TN, c->TNP, c->DN, c->CTN);
   hipLaunchKernelGGL(k_lut_tm3, dim3(1024), dim3(256), 0, st, d, c->TM, c->TM3);
   HIP_TRY(hipGetLastError());
   c->has_lut = true;
@@ -391,7 +395,7 @@ int lsim_forward_query(lsim_ctx* c, int64_t n_seg, int32_t n_events, size_t* ws_
   int rc = make_dev(c, nullptr, n_events, d);
   if (rc) return rc;
   Ws w = layout(nullptr, n_seg, d.n_cells);
-  layout_center(w, nullptr, n_seg, d.L);
+  layout_center(w, nullptr, n_seg);
   *ws_bytes = w.bytes;
   if (max_rows_bound) {
     const int64_t plane_pixels = (int64_t)n_events * d.ntpc * d.nx * d.ny;
@@ -412,7 +416,7 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
   if (rc) return rc;
   const int64_t n = seg->n;
   Ws w = layout(ws, n, d.n_cells);
-  layout_center(w, ws, n, d.L);
+  layout_center(w, ws, n);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small: need " + std::to_string(w.bytes) + " bytes");
   SegIn in = make_segin(seg);
   int* status = status_dev;
@@ -441,7 +445,7 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
 
 static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_seg, int64_t max_rows,
                              const lsim_fwd_out* out, hipStream_t st) {
-  Tables tb{c->TM3, c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
+  Tables tb{c->TM3, c->TM, c->TN, c->TNP, c->DM, c->DN, c->CTM, c->CTN};
   RowOut ro;
   ro.wfs = out->wfs;
   ro.adc = out->adc;
@@ -505,7 +509,7 @@ int lsim_forward_rows(lsim_ctx* c, const lsim_params* p, int64_t n_seg, int32_t 
   rc = make_dev(c, p, n_events, d);
   if (rc) return rc;
   Ws w = layout(ws, n_seg, d.n_cells);
-  layout_center(w, ws, n_seg, d.L);
+  layout_center(w, ws, n_seg);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
   // rows never exceed segments; the per-row workspace buffers hold max(n, 1) entries
   if (max_rows > (n_seg > 1 ? n_seg : 1)) max_rows = n_seg > 1 ? n_seg : 1;
@@ -629,7 +633,7 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
   double* dz = cv.take<double>(max_rows);
   double* partial = cv.take<double>(max_rows * kNGrad);
   AdjRec* adj = cv.take<AdjRec>(n);
-  Tables tb{c->TM3, c->TM, c->TN, c->DM, c->DN, c->CTM, c->CTN};
+  Tables tb{c->TM3, c->TM, c->TN, c->TNP, c->DM, c->DN, c->CTM, c->CTN};
   SegIn in = make_segin(seg);
   uint32_t *keys, *vals;
   sorted_buffers(w, d.n_cells, keys, vals);
