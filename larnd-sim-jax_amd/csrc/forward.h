@@ -120,6 +120,7 @@ __global__ void k_copy_count(const int* __restrict__ src, int64_t* __restrict__ 
 // and 0 elsewhere, kWinPad entries per series, so chunk c of lane l reads P[63 c + l] with no
 // bounds test (windows up to 3 * 63 - 2 = 187 samples).
 constexpr int kWinPad = 3 * kWave;
+constexpr int kXcd = 8;  // XCDs of an MI355X (workgroup i runs on XCD i % 8)
 
 struct Tables {
   const float3* TM3;  // [nm][cm][cm][kWinPad] padded (T_{m-1}, T_m, T_{m+1}) windows of the diffusion bins
@@ -363,27 +364,24 @@ __device__ __forceinline__ void store_window(float* accw, int lane, float v0, fl
 // several independent pairs in flight.
 struct FarLoads {
   float t0, t1, t2;
-  float2 D;
+  float D;  // boundary correction Delta of the pair (charge included)
 };
 
 __device__ __forceinline__ void far_load(const Dev& d, const Tables& tb, const FarRec& R, int a, int b, int lane,
-                                         FarLoads& q) {
-  const int L = d.L;
+                                         float delta, FarLoads& q) {
   const int cxN = offset_bin(R.uxy & 0xffff, a, d.nsb);
   const int cyN = offset_bin(R.uxy >> 16, b, d.nsb);
   const int sidx = cxN * d.ly + cyN;
   const float* TNp = tb.TNP + sidx * kWinPad + lane;
-  q.D = tb.DN[sidx * d.nt + R.k0];
+  q.D = delta;
   q.t0 = TNp[0];
   q.t1 = TNp[kChunk];
   q.t2 = TNp[2 * kChunk];
-  (void)L;
 }
 
 __device__ __forceinline__ void far_store(const Dev& d, const FarRec& R, const FarLoads& q, float* accw, int lane) {
   const float f = R.f, omf = 1.f - f;
-  const float delta = R.ne * (q.D.x - f * q.D.y);
-  store_window(accw, lane, q.t0, q.t1, q.t2, R.ne, omf, f, delta, d.nt - d.L - R.k0, d.L, d.nout);
+  store_window(accw, lane, q.t0, q.t1, q.t2, R.ne, omf, f, q.D, d.nt - d.L - R.k0, d.L, d.nout);
 }
 
 // |a|, |b| <= 1: the diffusion bins landing on this row (interpolated templates) plus the
@@ -391,8 +389,8 @@ __device__ __forceinline__ void far_store(const Dev& d, const FarRec& R, const F
 // T_{idx+1}) so one 12-byte load serves the three Lagrange nodes.  Bin i of the segment lies
 // delta_i = floor((ux + i - SYM) / nsb) in {-1, 0, 1} pixels from the main pixel.
 template <int NB, bool CENTER>
-__device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, float* accw,
-                                          int lane) {
+__device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, float delta,
+                                          float* accw, int lane) {
   constexpr int SYM = (NB - 1) / 2;
   const int nsb = d.nsb, L = d.L, cm = d.cm;
   const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
@@ -419,14 +417,11 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
   const int idx = R.idx, k0 = R.k0;
   const float f = R.f, omf = 1.f - f;
   const float la = R.la, lb = R.lb, lc = R.lc;
-  float delta = 0.f;
   float v0 = 0.f, v1 = 0.f, v2 = 0.f;
   if (has_nb) {
     const int cxN = offset_bin(ux, a, nsb);
     const int cyN = offset_bin(uy, b, nsb);
     const int64_t sidx = (int64_t)cxN * d.ly + cyN;
-    const float2 D = tb.DN[sidx * d.nt + k0];
-    delta = R.ne * (D.x - f * D.y);
     const float* TNp = tb.TNP + sidx * kWinPad + lane;
     const float ne = R.ne;
     v0 = ne * TNp[0];
@@ -444,10 +439,6 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
         for (int jj = 0; jj < NB; ++jj) {
           const float c = wxs[i] * wys[jj];
           const int64_t coff = (int64_t)cxs[i] * cm + cys[jj];
-          if (c != 0.f) {
-            const float2 D = tb.DM[((int64_t)idx * cm * cm + coff) * d.nt + k0];
-            delta += c * (D.x - f * D.y);
-          }
           const float3* tp = T3 + coff * kWinPad;
           const float3 q0 = tp[0];
           const float3 q1 = tp[kChunk];
@@ -463,13 +454,12 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
 }
 
 // Segment-parallel window of the diffusion bins that land on the segment's own (main) pixel:
-// vc[s][j] = sum_bins n_bin * (b T_idx + a T_{idx-1} + c T_{idx+1})[cx][cy][j] and its boundary
-// correction dc[s] (sim_jax.py:186-194, 236-247).  One wave per segment (sorted order); the
-// waveform kernel then adds these like a neighbour stripe.
+// vc[s][j] = sum_bins n_bin * (b T_idx + a T_{idx-1} + c T_{idx+1})[cx][cy][j]
+// (sim_jax.py:186-194), stored padded like the window tables.  One wave per segment (sorted
+// order); the waveform kernel then adds these like a neighbour stripe.
 template <int NB>
 __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* __restrict__ rec,
-                                                const uint32_t* __restrict__ keys, int64_t n, float* __restrict__ vc,
-                                                float* __restrict__ dc) {
+                                                const uint32_t* __restrict__ keys, int64_t n, float* __restrict__ vc) {
   constexpr int SYM = (NB - 1) / 2;
   const int lane = threadIdx.x & (kWave - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
@@ -492,7 +482,6 @@ __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* 
   const float f = R.f, la = R.la, lb = R.lb, lc = R.lc;
   const int idx = R.idx, k0 = R.k0;
   const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * kWinPad + lane;
-  float delta = 0.f;
   // padded positions p = lane, lane + 64, lane + 128 (kWinPad = 3 * 64)
   float v1 = 0.f, v2 = 0.f, v3 = 0.f;
 #pragma unroll
@@ -502,10 +491,6 @@ __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* 
       for (int jj = 0; jj < NB; ++jj) {
         const float c = wxs[i] * wys[jj];
         const int coff = cxs[i] * cm + cys[jj];
-        if (c != 0.f) {
-          const float2 D = tb.DM[((int64_t)idx * cm * cm + coff) * d.nt + k0];
-          delta += c * (D.x - f * D.y);
-        }
         const float3* tp = T3 + (int64_t)coff * kWinPad;
         const float3 q1 = tp[0], q2 = tp[kWave], q3 = tp[2 * kWave];
         v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
@@ -519,18 +504,75 @@ __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* 
   o[kWave] = v2;
   o[2 * kWave] = v3;
   (void)L;
-  if (lane == 0) dc[s] = delta;
 }
 
-// Center pair from the precomputed window: V = vc[s] (charge included), Delta = dc[s].
-__device__ __forceinline__ void center_load(const Dev& d, const float* __restrict__ vc, const float* __restrict__ dc,
+// Boundary corrections of every (segment, row offset) pair, one thread per pair:
+// dtot[ab][s] = n_e (D.x - f D.y) of the neighbour stripe (raw response, unless the offset is
+// (0, 0)) plus, for |a|, |b| <= 1, sum over the diffusion bins landing on that row of
+// n_bin (D.x - f D.y) of the idx template (sim_jax.py:227-261), D = (C[nt-L] - C[k0],
+// C[k0+1] - C[k0]).  ab = (a + n)(2n + 1) + (b + n).  Work units (ab, block of 256 segments)
+// run ab-major in contiguous XCD bands, so the D table rows one offset reads (100 sub-bin
+// positions x nt) stay in one XCD's L2; the waveform kernels then read dtot[ab][cell range]
+// contiguously instead of gathering D per pair.
+template <int NB>
+__global__ __launch_bounds__(256) void k_corr(Dev d, Tables tb, const SegRec* __restrict__ rec,
+                                              const uint32_t* __restrict__ keys, int64_t n, int64_t nblk,
+                                              float* __restrict__ dtot) {
+  constexpr int SYM = (NB - 1) / 2;
+  const int W = 2 * d.nn + 1;
+  const int64_t units = (int64_t)W * W * nblk;
+  const int64_t xcd = blockIdx.x % kXcd, j = blockIdx.x / kXcd;
+  const int64_t q = units / kXcd, rem = units % kXcd;
+  if (j >= q + (xcd < rem ? 1 : 0)) return;
+  const int64_t u = xcd * q + (xcd < rem ? xcd : rem) + j;
+  const int ab = (int)(u / nblk);
+  const int64_t s = (u % nblk) * 256 + threadIdx.x;
+  if (s >= n || keys[s] >= (uint32_t)d.n_cells) return;
+  const int a = ab / W - d.nn, b = ab % W - d.nn;
+  const int nsb = d.nsb, cm = d.cm;
+  const FarRec F = *reinterpret_cast<const FarRec*>(rec + s);
+  const int ux = F.uxy & 0xffff, uy = F.uxy >> 16;
+  const float f = F.f;
+  float delta = 0.f;
+  if (a != 0 || b != 0) {
+    const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
+    const float2 D = tb.DN[sidx * d.nt + F.k0];
+    delta = F.ne * (D.x - f * D.y);
+  }
+  if (a >= -1 && a <= 1 && b >= -1 && b <= 1) {
+    const SegRec R = rec[s];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int vx = ux + i - SYM;
+      const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
+      if (dx != a) continue;
+      const float wxs = R.ne * R.wx[i];
+      const int cx = offset_bin(vx - dx * nsb, 0, nsb);
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        const int vy = uy + jj - SYM;
+        const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
+        if (dy != b) continue;
+        const float c = wxs * R.wy[jj];
+        if (c != 0.f) {
+          const float2 D = tb.DM[(((int64_t)R.idx * cm + cx) * cm + offset_bin(vy - dy * nsb, 0, nsb)) * d.nt + F.k0];
+          delta += c * (D.x - f * D.y);
+        }
+      }
+    }
+  }
+  dtot[(int64_t)ab * n + s] = delta;
+}
+
+// Center pair from the precomputed window: V = vc[s] (charge included), Delta = dtc[s].
+__device__ __forceinline__ void center_load(const Dev& d, const float* __restrict__ vc, const float* __restrict__ dtc,
                                             int s, int lane, FarLoads& q) {
   const float* V = vc + (int64_t)s * kWinPad + lane;
   q.t0 = V[0];
   q.t1 = V[kChunk];
   q.t2 = V[2 * kChunk];
   (void)d;
-  q.D = make_float2(dc[s], 0.f);
+  q.D = dtc[s];
 }
 
 // Dynamic LDS bytes of the waveform kernel: the per-wave partial rows, reused afterwards for the
@@ -541,7 +583,6 @@ inline size_t rows_lds_bytes(int waves, int nout) {
 }
 
 constexpr int kMaxCells = 17 * 17;  // (2n+1)^2 for number_pix_neighbors <= 8
-constexpr int kXcd = 8;              // XCDs of an MI355X
 #ifndef LSIM_FAR_GROUP
 #define LSIM_FAR_GROUP 4
 #endif
@@ -551,8 +592,8 @@ template <int NB, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const int2* __restrict__ rng, const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
-                                                            const float* __restrict__ vc, const float* __restrict__ dc,
-                                                            RowOut out) {
+                                                            const float* __restrict__ vc, const float* __restrict__ dtot,
+                                                            int64_t nseg, RowOut out) {
   constexpr int NT = WAVES * kWave;
   extern __shared__ __align__(16) char smem[];
   __shared__ double shd[NT / kWave];
@@ -626,23 +667,28 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
     const int ab = __builtin_amdgcn_readfirstlane(c_ab[c]);
     const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+    const float* dt = dtot + (int64_t)((a + d.nn) * (2 * d.nn + 1) + (b + d.nn)) * nseg + beg;
     for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
       FarRec R[kFarGroup];
+      float D[kFarGroup];
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g) {
         const int j = k + g * WAVES;
-        R[g] = *reinterpret_cast<const FarRec*>(rec + beg + (j < cnt ? j : k));
+        const int jj = j < cnt ? j : k;
+        R[g] = *reinterpret_cast<const FarRec*>(rec + beg + jj);
+        D[g] = dt[jj];
         if (j >= cnt) R[g].ne = 0.f;
       }
       FarLoads q[kFarGroup];
 #pragma unroll
-      for (int g = 0; g < kFarGroup; ++g) far_load(d, tb, R[g], a, b, lane, q[g]);
+      for (int g = 0; g < kFarGroup; ++g) far_load(d, tb, R[g], a, b, lane, D[g], q[g]);
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g)
         if (R[g].ne != 0.f) far_store(d, R[g], q[g], accw, lane);
     }
   }
   // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
+  const float* dtc = dtot + (int64_t)(d.nn * (2 * d.nn + 1) + d.nn) * nseg;
   for (int c = nfc; c < n_center_end; ++c) {
     const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
     const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
@@ -658,11 +704,11 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
       }
       FarLoads q[kFarGroup];
 #pragma unroll
-      for (int g = 0; g < kFarGroup; ++g) center_load(d, vc, dc, sg[g], lane, q[g]);
+      for (int g = 0; g < kFarGroup; ++g) center_load(d, vc, dtc, sg[g], lane, q[g]);
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g)
         if (R[g].ne != 0.f)
-          store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, 1.f, 1.f - R[g].f, R[g].f, q[g].D.x, d.nt - d.L - R[g].k0,
+          store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, 1.f, 1.f - R[g].f, R[g].f, q[g].D, d.nt - d.L - R[g].k0,
                        d.L, d.nout);
     }
   }
@@ -675,7 +721,9 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
       const int ab = __builtin_amdgcn_readfirstlane(c_ab[cur]);
       const SegRec R = rec[s];
       if (R.ne == 0.f) continue;
-      near_pair<NB, false>(d, tb, R, (ab >> 8) - 64, (ab & 255) - 64, accw, lane);
+      const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+      const float delta = dtot[(int64_t)((a + d.nn) * (2 * d.nn + 1) + (b + d.nn)) * nseg + s];
+      near_pair<NB, false>(d, tb, R, a, b, delta, accw, lane);
     }
   }
   __syncthreads();

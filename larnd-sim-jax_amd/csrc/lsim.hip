@@ -174,8 +174,8 @@ struct Ws {
   int* hit_off;
   int* scan_tmp;
   int* n_hits32;
-  float* vc;  // [n][L] centre-pixel windows of each segment (sorted order)
-  float* dc;  // [n] their boundary corrections
+  float* vc;    // [n][kWinPad] centre-pixel windows of each segment (sorted order, padded)
+  float* dtot;  // [(2 nn + 1)^2][n] boundary corrections of every (segment, row offset) pair
   size_t bytes;
 };
 
@@ -199,17 +199,18 @@ Ws layout(void* base, int64_t n, int64_t n_cells) {
   w.scan_tmp = c.take<int>(scan_tiles(n) + 4);
   w.n_hits32 = c.take<int>(4);
   w.vc = nullptr;
-  w.dc = nullptr;
+  w.dtot = nullptr;
   w.bytes = c.off + 256;
   return w;
 }
 
-// The centre windows need kWinPad floats per segment (padded); carved after the rest.
-void layout_center(Ws& w, void* base, int64_t n) {
+// The centre windows (kWinPad floats per segment) and the pair corrections ((2 nn + 1)^2 floats
+// per segment); carved after the rest.
+void layout_center(Ws& w, void* base, int64_t n, int nn) {
   Carve c(base);
   c.off = w.bytes;
   w.vc = c.take<float>(n * kWinPad);
-  w.dc = c.take<float>(n);
+  w.dtot = c.take<float>((int64_t)(2 * nn + 1) * (2 * nn + 1) * n);
   w.bytes = c.off + 256;
 }
 
@@ -229,17 +230,19 @@ void sorted_buffers(const Ws& w, int64_t n_cells, uint32_t*& keys, uint32_t*& va
 }
 
 template <int NB, int WAVES>
-void launch_rows_nb(const Dev& d, const Tables& tb, const Ws& w, int64_t max_rows, const RowOut& ro, hipStream_t st) {
+void launch_rows_nb(const Dev& d, const Tables& tb, const Ws& w, int64_t n_seg, int64_t max_rows, const RowOut& ro,
+                    hipStream_t st) {
   hipLaunchKernelGGL((k_rows_fwd<NB, WAVES>), dim3((unsigned)max_rows), dim3(WAVES * kWave),
-                     rows_lds_bytes(WAVES, d.nout), st, d, tb, w.rec, w.rng, w.row_cell, w.n_rows, max_rows, w.vc, w.dc,
-                     ro);
+                     rows_lds_bytes(WAVES, d.nout), st, d, tb, w.rec, w.rng, w.row_cell, w.n_rows, max_rows, w.vc, w.dtot,
+                     n_seg, ro);
 }
 
 template <int WAVES>
-void launch_rows(const Dev& d, const Tables& tb, const Ws& w, int64_t max_rows, const RowOut& ro, hipStream_t st) {
-  if (d.nb == 5) launch_rows_nb<5, WAVES>(d, tb, w, max_rows, ro, st);
-  else if (d.nb == 3) launch_rows_nb<3, WAVES>(d, tb, w, max_rows, ro, st);
-  else launch_rows_nb<1, WAVES>(d, tb, w, max_rows, ro, st);
+void launch_rows(const Dev& d, const Tables& tb, const Ws& w, int64_t n_seg, int64_t max_rows, const RowOut& ro,
+                 hipStream_t st) {
+  if (d.nb == 5) launch_rows_nb<5, WAVES>(d, tb, w, n_seg, max_rows, ro, st);
+  else if (d.nb == 3) launch_rows_nb<3, WAVES>(d, tb, w, n_seg, max_rows, ro, st);
+  else launch_rows_nb<1, WAVES>(d, tb, w, n_seg, max_rows, ro, st);
 }
 
 int g_row_waves = 4;
@@ -395,7 +398,7 @@ int lsim_forward_query(lsim_ctx* c, int64_t n_seg, int32_t n_events, size_t* ws_
   int rc = make_dev(c, nullptr, n_events, d);
   if (rc) return rc;
   Ws w = layout(nullptr, n_seg, d.n_cells);
-  layout_center(w, nullptr, n_seg);
+  layout_center(w, nullptr, n_seg, d.nn);
   *ws_bytes = w.bytes;
   if (max_rows_bound) {
     const int64_t plane_pixels = (int64_t)n_events * d.ntpc * d.nx * d.ny;
@@ -416,7 +419,7 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
   if (rc) return rc;
   const int64_t n = seg->n;
   Ws w = layout(ws, n, d.n_cells);
-  layout_center(w, ws, n);
+  layout_center(w, ws, n, d.nn);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small: need " + std::to_string(w.bytes) + " bytes");
   SegIn in = make_segin(seg);
   int* status = status_dev;
@@ -458,18 +461,28 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_s
     uint32_t *keys, *vals;
     sorted_buffers(w, d.n_cells, keys, vals);
     const unsigned g = (unsigned)((n_seg + 3) / 4);
-    if (d.nb == 5) hipLaunchKernelGGL(k_center<5>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc, w.dc);
-    else if (d.nb == 3) hipLaunchKernelGGL(k_center<3>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc, w.dc);
-    else hipLaunchKernelGGL(k_center<1>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc, w.dc);
+    const int64_t nblk = (n_seg + 255) / 256;
+    const int64_t units = (int64_t)(2 * d.nn + 1) * (2 * d.nn + 1) * nblk;
+    const unsigned gc = (unsigned)(((units + kXcd - 1) / kXcd) * kXcd);
+    if (d.nb == 5) {
+      hipLaunchKernelGGL(k_center<5>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
+      hipLaunchKernelGGL(k_corr<5>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, nblk, w.dtot);
+    } else if (d.nb == 3) {
+      hipLaunchKernelGGL(k_center<3>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
+      hipLaunchKernelGGL(k_corr<3>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, nblk, w.dtot);
+    } else {
+      hipLaunchKernelGGL(k_center<1>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
+      hipLaunchKernelGGL(k_corr<1>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, nblk, w.dtot);
+    }
   }
   if (max_rows > 0) {
     hipEvent_t stop_ev;
     timer_begin(st, &stop_ev);
     switch (g_row_waves) {
-      case 1: launch_rows<1>(d, tb, w, max_rows, ro, st); break;
-      case 2: launch_rows<2>(d, tb, w, max_rows, ro, st); break;
-      case 8: launch_rows<8>(d, tb, w, max_rows, ro, st); break;
-      default: launch_rows<4>(d, tb, w, max_rows, ro, st); break;
+      case 1: launch_rows<1>(d, tb, w, n_seg, max_rows, ro, st); break;
+      case 2: launch_rows<2>(d, tb, w, n_seg, max_rows, ro, st); break;
+      case 8: launch_rows<8>(d, tb, w, n_seg, max_rows, ro, st); break;
+      default: launch_rows<4>(d, tb, w, n_seg, max_rows, ro, st); break;
     }
     HIP_TRY(hipGetLastError());
     if (stop_ev) HIP_TRY(hipEventRecord(stop_ev, st));
@@ -509,7 +522,7 @@ int lsim_forward_rows(lsim_ctx* c, const lsim_params* p, int64_t n_seg, int32_t 
   rc = make_dev(c, p, n_events, d);
   if (rc) return rc;
   Ws w = layout(ws, n_seg, d.n_cells);
-  layout_center(w, ws, n_seg);
+  layout_center(w, ws, n_seg, d.nn);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
   // rows never exceed segments; the per-row workspace buffers hold max(n, 1) entries
   if (max_rows > (n_seg > 1 ? n_seg : 1)) max_rows = n_seg > 1 ? n_seg : 1;
