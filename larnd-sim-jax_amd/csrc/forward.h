@@ -6,6 +6,26 @@
 
 namespace lsim {
 
+// Window tables used by the waveform kernels are zero padded: P[p] = W[p - 2] for 2 <= p < L + 2
+// and 0 elsewhere, kWinPad entries per series, so chunk c of lane l reads P[63 c + l] with no
+// bounds test (windows up to 3 * 63 - 2 = 187 samples).
+constexpr int kWinPad = 3 * kWave;
+constexpr int kXcd = 8;  // XCDs of an MI355X (workgroup i runs on XCD i % 8)
+
+// Per-segment record of the window-store paths (sorted order, 32 bytes = one scalar load): window
+// start, charge, fractional tick, the interior-window flag, and byte offsets from which the
+// padded raw-window row of any neighbour offset follows with a few scalar operations.  For a
+// sub-bin position u and a pixel offset a != 0, offset_bin(u, a) = nsb a + nsb/2 - 1 - u (a > 0)
+// or u - nsb/2 + nsb |a| (a < 0): affine in u, so the row is a per-cell constant plus +-U.
+struct __align__(32) PairRec {
+  int S;          // first output tick of the (1-f) window: nt - L - k0
+  float ne, f;
+  int interior;   // 2 <= S and S + L <= nout
+  int U, V;       // u * ly * kWinPad * 4 and v * kWinPad * 4
+  int U0, V0;     // offset_bin(u, 0) * ly * kWinPad * 4 and offset_bin(v, 0) * kWinPad * 4
+};
+static_assert(sizeof(PairRec) == 32, "PairRec layout");
+
 // ------------------------------------------------------------------------------------------
 // small utility kernels
 // ------------------------------------------------------------------------------------------
@@ -48,7 +68,7 @@ __global__ void k_keys(Dev d, SegIn in, uint32_t* __restrict__ keys, uint32_t* _
 // ------------------------------------------------------------------------------------------
 
 __global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
-                          SegRec* __restrict__ rec) {
+                          SegRec* __restrict__ rec, PairRec* __restrict__ prs) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || keys[i] >= (uint32_t)d.n_cells) return;
   Phys ph;
@@ -71,8 +91,19 @@ __global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, co
     r.wx[k] = k < d.nb ? (float)wx[k] : 0.f;
     r.wy[k] = k < d.nb ? (float)wy[k] : 0.f;
   }
-  r.uxy = pymod_i(ph.bx, d.nsb) | (pymod_i(ph.by, d.nsb) << 16);
+  const int ux = pymod_i(ph.bx, d.nsb), uy = pymod_i(ph.by, d.nsb);
+  r.uxy = ux | (uy << 16);
   rec[i] = r;
+  PairRec P;
+  P.S = d.nt - d.L - ph.k0;
+  P.ne = r.ne;
+  P.f = r.f;
+  P.interior = (P.S >= 2 && P.S + d.L <= d.nout) ? 1 : 0;
+  P.U = ux * d.ly * kWinPad * 4;
+  P.V = uy * kWinPad * 4;
+  P.U0 = offset_bin(ux, 0, d.nsb) * d.ly * kWinPad * 4;
+  P.V0 = offset_bin(uy, 0, d.nsb) * kWinPad * 4;
+  prs[i] = P;
 }
 
 // Cell ranges [begin, end) in the sorted order and the "first segment of an on-plane cell" flag
@@ -116,11 +147,6 @@ __global__ void k_copy_count(const int* __restrict__ src, int64_t* __restrict__ 
 // 3. the waveform kernel: one workgroup owns one pixel row
 // ------------------------------------------------------------------------------------------
 
-// Window tables used by the waveform kernels are zero padded: P[p] = W[p - 2] for 2 <= p < L + 2
-// and 0 elsewhere, kWinPad entries per series, so chunk c of lane l reads P[63 c + l] with no
-// bounds test (windows up to 3 * 63 - 2 = 187 samples).
-constexpr int kWinPad = 3 * kWave;
-constexpr int kXcd = 8;  // XCDs of an MI355X (workgroup i runs on XCD i % 8)
 
 struct Tables {
   const float3* TM3;  // [nm][cm][cm][kWinPad] padded (T_{m-1}, T_m, T_{m+1}) windows of the diffusion bins
@@ -331,13 +357,13 @@ __device__ __forceinline__ void scatter_chunk(float* accw, int lane, int m, floa
 }
 
 // Store of a whole window held as three chunks (v0, v1, v2) of up to 63 outputs each, V = v
-// scaled by the charge: ne_omf = scale*(1-f), ne_f = scale*f.  `interior` windows
-// (2 <= S and S + L <= nout) need no per-lane tick checks: output m of chunk c goes to
+// scaled by the charge: ne_omf = scale*(1-f), ne_f = scale*f.  `interior` windows (the caller
+// passes 2 <= S and S + L <= nout) need no per-lane tick checks: output m of chunk c goes to
 // accw[S + m] with m = 63 c + lane - 2.
 __device__ __forceinline__ void store_window(float* accw, int lane, float v0, float v1, float v2, float scale, float omf,
-                                             float f, float delta, int S, int L, int nout) {
+                                             float f, float delta, int S, int L, int nout, bool interior) {
   const float n0 = lane_next(v0), n1 = lane_next(v1), n2 = lane_next(v2);
-  if (S >= 2 && S + L <= nout) {
+  if (interior) {
     // all three chunks in one go (disjoint addresses); lanes past the window add +0 to the
     // row or to the kRowPad padding after it, so no per-chunk branch is needed
     const float so = scale * omf, sf = scale * f;
@@ -367,21 +393,18 @@ struct FarLoads {
   float D;  // boundary correction Delta of the pair (charge included)
 };
 
-__device__ __forceinline__ void far_load(const Dev& d, const Tables& tb, const FarRec& R, int a, int b, int lane,
-                                         float delta, FarLoads& q) {
-  const int cxN = offset_bin(R.uxy & 0xffff, a, d.nsb);
-  const int cyN = offset_bin(R.uxy >> 16, b, d.nsb);
-  const int sidx = cxN * d.ly + cyN;
-  const float* TNp = tb.TNP + sidx * kWinPad + lane;
+// Window loads of a neighbour-only pair: tnp = padded raw windows + this lane's byte offset + the
+// cell constant; off = the segment's part of the row offset (bytes).
+__device__ __forceinline__ void far_load(const char* __restrict__ tnp, int off, float delta, FarLoads& q) {
+  const float* t = reinterpret_cast<const float*>(tnp + off);
   q.D = delta;
-  q.t0 = TNp[0];
-  q.t1 = TNp[kChunk];
-  q.t2 = TNp[2 * kChunk];
+  q.t0 = t[0];
+  q.t1 = t[kChunk];
+  q.t2 = t[2 * kChunk];
 }
 
-__device__ __forceinline__ void far_store(const Dev& d, const FarRec& R, const FarLoads& q, float* accw, int lane) {
-  const float f = R.f, omf = 1.f - f;
-  store_window(accw, lane, q.t0, q.t1, q.t2, R.ne, omf, f, q.D, d.nt - d.L - R.k0, d.L, d.nout);
+__device__ __forceinline__ void far_store(const Dev& d, const PairRec& P, const FarLoads& q, float* accw, int lane) {
+  store_window(accw, lane, q.t0, q.t1, q.t2, P.ne, 1.f - P.f, P.f, q.D, P.S, d.L, d.nout, P.interior != 0);
 }
 
 // |a|, |b| <= 1: the diffusion bins landing on this row (interpolated templates) plus the
@@ -450,7 +473,8 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
       }
     }
   }
-  store_window(accw, lane, v0, v1, v2, 1.f, omf, f, delta, d.nt - L - k0, L, d.nout);
+  const int S = d.nt - L - k0;
+  store_window(accw, lane, v0, v1, v2, 1.f, omf, f, delta, S, L, d.nout, S >= 2 && S + L <= d.nout);
 }
 
 // Segment-parallel window of the diffusion bins that land on the segment's own (main) pixel:
@@ -593,7 +617,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
                                                             const int2* __restrict__ rng, const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
                                                             const float* __restrict__ vc, const float* __restrict__ dtot,
-                                                            int64_t nseg, RowOut out) {
+                                                            const PairRec* __restrict__ prs, int64_t nseg, RowOut out) {
   constexpr int NT = WAVES * kWave;
   extern __shared__ __align__(16) char smem[];
   __shared__ double shd[NT / kWave];
@@ -668,23 +692,35 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     const int ab = __builtin_amdgcn_readfirstlane(c_ab[c]);
     const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
     const float* dt = dtot + (int64_t)((a + d.nn) * (2 * d.nn + 1) + (b + d.nn)) * nseg + beg;
+    const PairRec* pr = prs + beg;
+    // row offset of the pair = cell constant + ma U + za U0 + mb V + zb V0 (bytes)
+    const int W4 = kWinPad * 4, nsb = d.nsb;
+    int kc = 0, ma = 0, za = 0, mb = 0, zb = 0;
+    if (a > 0) { kc += (nsb * a + nsb / 2 - 1) * d.ly * W4; ma = -1; }
+    else if (a < 0) { kc += (nsb * -a - nsb / 2) * d.ly * W4; ma = 1; }
+    else za = 1;
+    if (b > 0) { kc += (nsb * b + nsb / 2 - 1) * W4; mb = -1; }
+    else if (b < 0) { kc += (nsb * -b - nsb / 2) * W4; mb = 1; }
+    else zb = 1;
+    const char* tnp = reinterpret_cast<const char*>(tb.TNP + lane) + kc;
     for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
-      FarRec R[kFarGroup];
+      PairRec P[kFarGroup];
       float D[kFarGroup];
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g) {
         const int j = k + g * WAVES;
         const int jj = j < cnt ? j : k;
-        R[g] = *reinterpret_cast<const FarRec*>(rec + beg + jj);
+        P[g] = pr[jj];
         D[g] = dt[jj];
-        if (j >= cnt) R[g].ne = 0.f;
+        if (j >= cnt) P[g].ne = 0.f;
       }
       FarLoads q[kFarGroup];
 #pragma unroll
-      for (int g = 0; g < kFarGroup; ++g) far_load(d, tb, R[g], a, b, lane, D[g], q[g]);
+      for (int g = 0; g < kFarGroup; ++g)
+        far_load(tnp, ma * P[g].U + za * P[g].U0 + mb * P[g].V + zb * P[g].V0, D[g], q[g]);
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g)
-        if (R[g].ne != 0.f) far_store(d, R[g], q[g], accw, lane);
+        if (P[g].ne != 0.f) far_store(d, P[g], q[g], accw, lane);
     }
   }
   // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
@@ -693,23 +729,23 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
     const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
     for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
-      FarRec R[kFarGroup];
+      PairRec P[kFarGroup];
       int sg[kFarGroup];
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g) {
         const int j = k + g * WAVES;
         sg[g] = beg + (j < cnt ? j : k);
-        R[g] = *reinterpret_cast<const FarRec*>(rec + sg[g]);
-        if (j >= cnt) R[g].ne = 0.f;
+        P[g] = prs[sg[g]];
+        if (j >= cnt) P[g].ne = 0.f;
       }
       FarLoads q[kFarGroup];
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g) center_load(d, vc, dtc, sg[g], lane, q[g]);
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g)
-        if (R[g].ne != 0.f)
-          store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, 1.f, 1.f - R[g].f, R[g].f, q[g].D, d.nt - d.L - R[g].k0,
-                       d.L, d.nout);
+        if (P[g].ne != 0.f)
+          store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, 1.f, 1.f - P[g].f, P[g].f, q[g].D, P[g].S, d.L, d.nout,
+                       P[g].interior != 0);
     }
   }
   {

@@ -164,6 +164,7 @@ struct Ws {
   uint32_t *k0, *v0, *k1, *v1;  // sort ping-pong
   void* sort_scratch;
   SegRec* rec;
+  PairRec* prs;  // [n] window-store records (sorted order)
   int2* rng;
   int* cell_row;
   int* flag;
@@ -188,6 +189,7 @@ Ws layout(void* base, int64_t n, int64_t n_cells) {
   w.v1 = c.take<uint32_t>(n);
   w.sort_scratch = c.take<char>((int64_t)radix_sort_scratch(n));
   w.rec = c.take<SegRec>(n);
+  w.prs = c.take<PairRec>(n);
   w.rng = c.take<int2>(n_cells);
   w.cell_row = c.take<int>(n_cells);
   w.flag = c.take<int>(n);
@@ -234,7 +236,7 @@ void launch_rows_nb(const Dev& d, const Tables& tb, const Ws& w, int64_t n_seg, 
                     hipStream_t st) {
   hipLaunchKernelGGL((k_rows_fwd<NB, WAVES>), dim3((unsigned)max_rows), dim3(WAVES * kWave),
                      rows_lds_bytes(WAVES, d.nout), st, d, tb, w.rec, w.rng, w.row_cell, w.n_rows, max_rows, w.vc, w.dtot,
-                     n_seg, ro);
+                     w.prs, n_seg, ro);
 }
 
 template <int WAVES>
@@ -436,7 +438,7 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
     HIP_TRY(err);
     uint32_t *keys, *vals;
     sorted_buffers(w, d.n_cells, keys, vals);
-    hipLaunchKernelGGL(k_records, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, w.rec);
+    hipLaunchKernelGGL(k_records, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, w.rec, w.prs);
     hipLaunchKernelGGL(k_cells, dim3(g), dim3(256), 0, st, d, keys, n, w.rng, w.flag);
     HIP_TRY((exclusive_scan<int, int>(w.flag, w.ridx, n, w.scan_tmp, w.n_rows, st)));
     hipLaunchKernelGGL(k_rows_map, dim3(g), dim3(256), 0, st, d, keys, w.flag, w.ridx, n, w.row_cell, w.cell_row);
