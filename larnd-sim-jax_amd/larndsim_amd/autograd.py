@@ -25,12 +25,16 @@ class LUTSimFunction(torch.autograd.Function):
         ctx.hit_index = h["hit_index"].long()
         ctx.theta_device = theta.device
         ctx.set_materialize_grads(False)
-        others = dict(h)
+        # the non-differentiable outputs, kept for the caller; the differentiable ones (adcs,
+        # pixel_z) are not stored here: a reference from the context back to its own outputs
+        # would keep every step's workspace alive until the cycle collector runs
+        others = {k: v for k, v in h.items() if k not in ("adcs", "pixel_z")}
         others["unique_pixels"] = st.out["pixel_ids"][: st.max_rows]
         others["wfs"] = st.out["wfs"][: st.max_rows]
         ctx.others = others
-        ctx.mark_non_differentiable(others["wfs"])
-        return h["adcs"], h["pixel_z"], others["wfs"]
+        wfs = st.out["wfs"][: st.max_rows]  # a view object of its own (not the one kept above)
+        ctx.mark_non_differentiable(wfs)
+        return h["adcs"], h["pixel_z"], wfs
 
     @staticmethod
     def backward(ctx, g_adc, g_z, g_wfs):
@@ -48,6 +52,7 @@ class LUTSimFunction(torch.autograd.Function):
             d_z = torch.zeros(n, dtype=torch.float32, device=dev)
             d_z.index_copy_(0, ctx.hit_index, g_z.to(torch.float32))
         grad = ctx.sim.backward(st, d_adc, d_z).to(ctx.theta_device)
+        ctx.state = None  # the workspace and tape are not needed again (no double backward)
         return grad, None, None, None, None, None
 
 

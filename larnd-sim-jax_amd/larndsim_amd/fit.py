@@ -192,10 +192,10 @@ class GradientDescentFitter:
         pred = self.sim_strategy.predict(params, tracks, fields, None)
         loss, aux = self.loss_strategy.compute(params, pred, target)
         if not with_grad:
-            return float(loss), None, aux
+            return float(loss.detach()), None, aux
         loss.backward()
         grads = {k: float(t.grad) if t.grad is not None else 0.0 for k, t in norm_t.items()}
-        return float(loss), grads, aux
+        return float(loss.detach()), grads, aux
 
     def process_grads(self, grads: dict) -> dict:
         if any(math.isnan(g) for g in grads.values()):

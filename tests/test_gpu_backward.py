@@ -87,3 +87,32 @@ def test_autograd_function(params4, lut, c1, frozen):
     mask = np.abs(g_ref) > 0
     assert _rel(g[mask], g_ref[mask]).max() < GRAD_TOL
     assert g[4] == 0.0
+
+
+def test_autograd_releases_workspace(params4, lut, c1):
+    """Repeated predict + backward keeps device memory flat without the cycle collector: the
+    autograd context holds no reference to its own outputs and drops the forward state after
+    the backward."""
+    import gc
+
+    from larndsim_amd.consts import GRAD_PARAMS
+    from larndsim_amd.strategies import HipLUTSimulation
+
+    tr, f = c1
+    strat = HipLUTSimulation(lut)
+    gc.collect()
+    gc.disable()
+    try:
+        used = []
+        for _ in range(6):
+            theta = {n: torch.tensor(float(getattr(params4, n)), dtype=torch.float64, requires_grad=True)
+                     for n in GRAD_PARAMS}
+            pred = strat.predict(params4.replace(**theta), tr, f, 0)
+            pred["adcs"].sum().backward()
+            del pred, theta
+            torch.cuda.synchronize()
+            used.append(torch.cuda.memory_allocated())
+        print("allocated after each step:", used)
+        assert used[-1] <= used[1]
+    finally:
+        gc.enable()
