@@ -182,6 +182,15 @@ int lsim_backward(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg,
                   size_t ws_bytes, int64_t max_rows, const lsim_fwd_out *fwd, const float *d_adc, const float *d_z,
                   void *bwd_ws, size_t bwd_ws_bytes, double *d_params, void *stream);
 
+/* Fit loss: event-blocked Gaussian-kernel sums of the hit-cloud MMD in mse_adc
+ * (src/larndsim/losses_jax.py:13-39 rbf_kernel / mmd, :58-80 mse_adc), without forming the
+ * kernel matrices: out[i][k] = sum_{j = lo[i]}^{hi[i]-1} exp(-|x_i - y_j|^2 / (2 sigma^2)) c[j][k].
+ * x [nx][4] and y [ny][4] float32 (x, y, z, unused), c [ny][4] float32 coefficients, lo/hi [nx]
+ * int32 with 0 <= lo[i] <= hi[i] <= ny (the points of x_i's event in y, both clouds sorted by
+ * event), out [nx][4] float64.  Distances and exponentials in float32, sums in float64. */
+int lsim_rbf_sums(const float *x, int64_t nx, const int32_t *lo, const int32_t *hi, const float *y, const float *c,
+                  int64_t ny, float sigma, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,39 @@
+// Event-blocked Gaussian-kernel sums for the hit-cloud MMD of the fit loss (losses_jax.py:13-39,
+// 58-80): out[i][k] = sum_{j = lo_i .. hi_i - 1} exp(-|x_i - y_j|^2 * inv) * c[j][k], k < 4.
+// The caller sorts both clouds by event and gives each point of x the index range of its event
+// in y, so no kernel value between events is ever formed (the reference's 1e5 cm event offset
+// makes those exactly zero).  Threads of a wave almost always share an event, so the y point and
+// its coefficients are wave-uniform loads; distances and the exponential are float32, the sums
+// float64.  Fixed summation order: bitwise reproducible.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lsim {
+
+__global__ __launch_bounds__(256) void k_rbf_sums(const float4* __restrict__ x, int64_t nx, const int32_t* __restrict__ lo,
+                                                  const int32_t* __restrict__ hi, const float4* __restrict__ y,
+                                                  const float4* __restrict__ c, float inv, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  const float4 p = x[i];
+  const int j0 = lo[i], j1 = hi[i];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int j = j0; j < j1; ++j) {
+    const float4 q = y[j];
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float k = __expf(-(dx * dx + dy * dy + dz * dz) * inv);
+    const float4 w = c[j];
+    s0 += (double)(k * w.x);
+    s1 += (double)(k * w.y);
+    s2 += (double)(k * w.z);
+    s3 += (double)(k * w.w);
+  }
+  double* o = out + i * 4;
+  o[0] = s0;
+  o[1] = s1;
+  o[2] = s2;
+  o[3] = s3;
+}
+
+}  // namespace lsim

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "backward.h"
+#include "loss.h"
 
 using namespace lsim;
 
@@ -669,6 +670,20 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
                          max_rows, pieces, partial);
   }
   hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(256), 0, st, d, partial, max_rows, dz, fwd->n_rows, max_rows, d_params);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+int lsim_rbf_sums(const float* x, int64_t nx, const int32_t* lo, const int32_t* hi, const float* y, const float* c,
+                  int64_t ny, float sigma, double* out, void* stream) {
+  if (nx < 0 || ny < 0) return fail(LSIM_E_ARG, "negative size");
+  if (!(sigma > 0.f)) return fail(LSIM_E_ARG, "sigma must be > 0");
+  if (nx == 0) return LSIM_OK;
+  if (!x || !lo || !hi || !out || (ny > 0 && (!y || !c))) return fail(LSIM_E_ARG, "NULL argument");
+  if (nx >= ((int64_t)1 << 40)) return fail(LSIM_E_ARG, "too many points");
+  const float inv = 1.f / (2.f * sigma * sigma);
+  hipLaunchKernelGGL(k_rbf_sums, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (const float4*)x, nx, lo, hi, (const float4*)y, (const float4*)c, inv, out);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }

@@ -18,7 +18,8 @@ MAX_ADC = 16
 EXPORTS = (
     "lsim_last_error", "lsim_version", "lsim_tape_bytes", "lsim_ctx_create", "lsim_ctx_destroy", "lsim_lut_set",
     "lsim_forward_query", "lsim_forward_prepare", "lsim_forward_rows", "lsim_forward", "lsim_fee_query",
-    "lsim_fee_forward", "lsim_backward_query", "lsim_backward", "lsim_set_row_waves", "lsim_segment_records", "lsim_timing", "lsim_timing_read",
+    "lsim_fee_forward", "lsim_backward_query", "lsim_backward", "lsim_set_row_waves", "lsim_segment_records", "lsim_timing",
+    "lsim_timing_read", "lsim_rbf_sums",
 )
 
 
@@ -98,6 +99,7 @@ def load(path: str = LIB_PATH):
         "lsim_segment_records": [vp, i64, i32, vp, sz, vp, vp, vp, vp],
         "lsim_timing": [C.c_int],
         "lsim_timing_read": [C.POINTER(C.c_double), C.POINTER(i64)],
+        "lsim_rbf_sums": [vp, i64, vp, vp, vp, vp, i64, C.c_float, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

@@ -63,6 +63,44 @@ def _kernel_rows(a, b, inv, c):
     return (torch.exp_(d.mul_(-inv)).double() @ c)
 
 
+def _pad4(a):
+    """(N, 3) -> contiguous (N, 4) float32 (fourth column zero)."""
+    out = torch.zeros((a.shape[0], 4), dtype=torch.float32, device=a.device)
+    out[:, :3] = a
+    return out
+
+
+def _rbf_sums_hip(xs, lo, hi, ys, c, sigma):
+    """out[i] = sum_{j=lo_i}^{hi_i-1} exp(-|xs_i - ys_j|^2 / 2 sigma^2) c_j on the GPU (lsim_rbf_sums)."""
+    import ctypes as C
+
+    from . import _lib
+
+    lib = _lib.load()
+    xs4, ys4 = _pad4(xs), _pad4(ys)
+    c4 = c.to(torch.float32).contiguous()
+    lo, hi = lo.to(torch.int32).contiguous(), hi.to(torch.int32).contiguous()
+    out = torch.empty((xs.shape[0], 4), dtype=torch.float64, device=xs.device)
+    st = C.c_void_p(torch.cuda.current_stream(xs.device).cuda_stream)
+    _lib.check(lib.lsim_rbf_sums(C.c_void_p(xs4.data_ptr()), xs.shape[0], C.c_void_p(lo.data_ptr()),
+                                 C.c_void_p(hi.data_ptr()), C.c_void_p(ys4.data_ptr()), C.c_void_p(c4.data_ptr()),
+                                 ys.shape[0], C.c_float(float(sigma)), C.c_void_p(out.data_ptr()), st))
+    return out
+
+
+def _block_sums(x, ex, y, ey, c, sigma):
+    """Per point of x: kernel-weighted sums of the 4 columns of c over the points of y in the same
+    event, on the GPU with the HIP kernel (no host synchronisation)."""
+    ox = torch.argsort(ex, stable=True)
+    oy = torch.argsort(ey, stable=True)
+    exs, eys = ex[ox].contiguous(), ey[oy].contiguous()
+    lo = torch.searchsorted(eys, exs, right=False)
+    hi = torch.searchsorted(eys, exs, right=True)
+    out = torch.empty((x.shape[0], 4), dtype=torch.float64, device=x.device)
+    out[ox] = _rbf_sums_hip(x[ox], lo, hi, y[oy], c[oy], sigma)
+    return out
+
+
 class _BlockMMDTerms(torch.autograd.Function):
     """Kernel sums of the event-blocked RBF MMD without keeping the kernel matrices.
 
@@ -72,7 +110,8 @@ class _BlockMMDTerms(torch.autograd.Function):
     Rows are processed in chunks per event; the backward reuses the per-point sums (K px,
     K (px x), K_xy py, K_xy (py y)) kept from the forward.  Gradients flow to x and px (the
     simulated hits); the target cloud is a constant.  Kernel values are evaluated in float32
-    (relative error ~1e-7 per term), sums in float64.
+    (relative error ~1e-7 per term), sums in float64.  On the GPU the sums come from the HIP
+    kernel lsim_rbf_sums (one pass, no host synchronisation); CPU tensors use chunked torch code.
     """
 
     @staticmethod
@@ -81,6 +120,14 @@ class _BlockMMDTerms(torch.autograd.Function):
         x, px, y, py = x.to(dt), px.to(dt), y.to(dt), py.to(dt)
         inv = 1.0 / (2.0 * sigma * sigma)
         N = x.shape[0]
+        if x.is_cuda:
+            kx = _block_sums(x, ex, x, ex, torch.cat([px[:, None], px[:, None] * x], dim=1), sigma)
+            ky = _block_sums(x, ex, y, ey, torch.cat([py[:, None], py[:, None] * y], dim=1), sigma)
+            A = (px * kx[:, 0]).sum()
+            C = (px * ky[:, 0]).sum()
+            ctx.save_for_backward(x, px, kx, ky)
+            ctx.inv = inv
+            return A, C
         kx = torch.zeros((N, 4), dtype=dt, device=x.device)  # [K_xx px, K_xx (px x)]
         ky = torch.zeros((N, 4), dtype=dt, device=x.device)  # [K_xy py, K_xy (py y)]
         ox, sx_ = _event_slices(ex)
@@ -122,6 +169,10 @@ def _target_term(y, py, ey, sigma, chunk):
     """sum K(y, y) py py within events (no gradient)."""
     inv = 1.0 / (2.0 * sigma * sigma)
     y, py = y.to(torch.float64), py.to(torch.float64)
+    if y.is_cuda:
+        c = torch.zeros((y.shape[0], 4), dtype=torch.float64, device=y.device)
+        c[:, 0] = py
+        return (py * _block_sums(y, ey, y, ey, c, sigma)[:, 0]).sum()
     oy, sy_ = _event_slices(ey)
     B = torch.zeros((), dtype=torch.float64, device=y.device)
     for _, (b0, b1) in sy_.items():

@@ -77,3 +77,37 @@ def test_fee_on_oracle_waveforms(sim, params4, lut, c1, tables):
     # the garbage row (-1) yields no compacted hit
     hp = out["hit_pixel"][: int(out["n_hits"].item())].cpu().numpy()
     assert (hp >= 0).all()
+
+
+def test_rbf_sums_kernel_matches_torch():
+    """lsim_rbf_sums (event-blocked Gaussian-kernel sums of the fit loss) against the chunked
+    torch evaluation of the same sums on the CPU."""
+    from larndsim_amd import losses
+
+    g = torch.Generator().manual_seed(3)
+    nx, ny = 3000, 2500
+    ex = torch.randint(0, 7, (nx,), generator=g)
+    ey = torch.randint(0, 7, (ny,), generator=g)
+    x = torch.randn(nx, 3, generator=g, dtype=torch.float64) * 5
+    y = torch.randn(ny, 3, generator=g, dtype=torch.float64) * 5
+    c = torch.rand(ny, 4, generator=g, dtype=torch.float64)
+    ref = torch.zeros(nx, 4, dtype=torch.float64)
+    for e in range(7):
+        ix, iy = (ex == e).nonzero().flatten(), (ey == e).nonzero().flatten()
+        if len(ix) and len(iy):
+            ref[ix] = torch.exp(-torch.cdist(x[ix], y[iy]).square() / (2 * 1.5 ** 2)) @ c[iy]
+    got = losses._block_sums(x.cuda(), ex.cuda(), y.cuda(), ey.cuda(), c.cuda(), 1.5).cpu()
+    torch.testing.assert_close(got, ref, rtol=2e-5, atol=1e-6)
+    # the MMD with gradients on the GPU equals the CPU path
+    px = torch.rand(nx, generator=g, dtype=torch.float64)
+    py = torch.rand(ny, generator=g, dtype=torch.float64)
+    xg = x.cuda().requires_grad_(True)
+    pg = px.cuda().requires_grad_(True)
+    mg = losses.mmd_blockwise(xg, y.cuda(), pg, py.cuda(), 1.5, ex.cuda(), ey.cuda())
+    mg.backward()
+    xc, pcpu = x.clone().requires_grad_(True), px.clone().requires_grad_(True)
+    mc = losses.mmd_blockwise(xc, y, pcpu, py, 1.5, ex, ey)
+    mc.backward()
+    assert float(mg) == pytest.approx(float(mc), rel=1e-5)
+    torch.testing.assert_close(xg.grad.cpu(), xc.grad, rtol=1e-4, atol=1e-8)
+    torch.testing.assert_close(pg.grad.cpu(), pcpu.grad, rtol=1e-4, atol=1e-8)
