@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-bwd", action="store_true")
-    ap.add_argument("--no-fit", action="store_true", help="skip the batched forward+backward of 32 events")
+    ap.add_argument("--no-batched-bwd", action="store_true", help="skip the batched forward+backward of 32 events")
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--fit-steps", type=int, default=200, help="Adam steps of the 32-event gradient fit (0: skip)")
     ap.add_argument("--copy-calibrate", action="store_true")
@@ -203,7 +203,7 @@ def main():
         del brun, btr, g
 
     # ---- batched forward + backward: 32 uneven events, one graph replay, vs per-event FD ----
-    if not a.no_fit:
+    if not a.no_batched_bwd:
         log("[bench] generating 32 events (20k-200k segments)")
         ft, ff = synthetic.workload("c5")
         NF = ft.shape[0]

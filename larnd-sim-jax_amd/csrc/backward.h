@@ -360,26 +360,40 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
   }
 }
 
-__global__ __launch_bounds__(256) void k_grad_reduce(Dev d, const double* __restrict__ partial, int64_t nblocks,
-                                                     const double* __restrict__ dz_part, const int64_t* __restrict__ n_rows_dev,
-                                                     int64_t max_rows, double* __restrict__ out) {
-  __shared__ double sh[4];
+// One pass over the per-row partials (all 7 gradients and the hit-z term per thread), then a
+// fixed-order combine: butterfly within each wave, waves in index order.  Deterministic.
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void k_grad_reduce(Dev d, const double* __restrict__ partial, int64_t nblocks,
+                                                                const double* __restrict__ dz_part,
+                                                                const int64_t* __restrict__ n_rows_dev, int64_t max_rows,
+                                                                double* __restrict__ out) {
+  __shared__ double sh[kNGrad + 1][kReduceThreads / kWave];
   int64_t P = *n_rows_dev;
   if (P > max_rows) P = max_rows;
-  for (int k = 0; k < kNGrad + 1; ++k) {
-    double s = 0.0;
-    if (k < kNGrad) {
-      for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[b * kNGrad + k];
-    } else {
-      for (int64_t r = threadIdx.x; r < P; r += blockDim.x) s += dz_part[r];
+  double acc[kNGrad + 1];
+#pragma unroll
+  for (int k = 0; k <= kNGrad; ++k) acc[k] = 0.0;
+  for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < kNGrad; ++k) acc[k] += partial[b * kNGrad + k];
+  }
+  for (int64_t r = threadIdx.x; r < P; r += blockDim.x) acc[kNGrad] += dz_part[r];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k <= kNGrad; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) sh[k][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot[kNGrad + 1];
+    for (int k = 0; k <= kNGrad; ++k) {
+      tot[k] = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot[k] += sh[k][w];
     }
-    double tot;
-    (void)block_exclusive_scan<double>(s, sh, tot);
-    if (threadIdx.x == 0) {
-      if (k < kNGrad) out[k] = tot;
-      else out[2] += tot * d.dv;  // hit z depends on eField through the drift velocity
-    }
-    __syncthreads();
+    for (int k = 0; k < kNGrad; ++k) out[k] = tot[k];
+    out[2] += tot[kNGrad] * d.dv;  // hit z depends on eField through the drift velocity
   }
 }
 

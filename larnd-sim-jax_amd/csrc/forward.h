@@ -20,7 +20,7 @@ constexpr int kXcd = 8;  // XCDs of an MI355X (workgroup i runs on XCD i % 8)
 struct __align__(32) PairRec {
   int S;          // first output tick of the (1-f) window: nt - L - k0
   float ne, f;
-  int interior;   // 2 <= S and S + L <= nout
+  int flags;      // bit 0: 2 <= S and S + L <= nout (interior window); bits 8-15: u; bits 16-23: v
   int U, V;       // u * ly * kWinPad * 4 and v * kWinPad * 4
   int U0, V0;     // offset_bin(u, 0) * ly * kWinPad * 4 and offset_bin(v, 0) * kWinPad * 4
 };
@@ -98,7 +98,7 @@ __global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, co
   P.S = d.nt - d.L - ph.k0;
   P.ne = r.ne;
   P.f = r.f;
-  P.interior = (P.S >= 2 && P.S + d.L <= d.nout) ? 1 : 0;
+  P.flags = ((P.S >= 2 && P.S + d.L <= d.nout) ? 1 : 0) | (ux << 8) | (uy << 16);
   P.U = ux * d.ly * kWinPad * 4;
   P.V = uy * kWinPad * 4;
   P.U0 = offset_bin(ux, 0, d.nsb) * d.ly * kWinPad * 4;
@@ -404,7 +404,7 @@ __device__ __forceinline__ void far_load(const char* __restrict__ tnp, int off, 
 }
 
 __device__ __forceinline__ void far_store(const Dev& d, const PairRec& P, const FarLoads& q, float* accw, int lane) {
-  store_window(accw, lane, q.t0, q.t1, q.t2, P.ne, 1.f - P.f, P.f, q.D, P.S, d.L, d.nout, P.interior != 0);
+  store_window(accw, lane, q.t0, q.t1, q.t2, P.ne, 1.f - P.f, P.f, q.D, P.S, d.L, d.nout, (P.flags & 1) != 0);
 }
 
 // |a|, |b| <= 1: the diffusion bins landing on this row (interpolated templates) plus the
@@ -540,8 +540,8 @@ __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* 
 // contiguously instead of gathering D per pair.
 template <int NB>
 __global__ __launch_bounds__(256) void k_corr(Dev d, Tables tb, const SegRec* __restrict__ rec,
-                                              const uint32_t* __restrict__ keys, int64_t n, int64_t nblk,
-                                              float* __restrict__ dtot) {
+                                              const PairRec* __restrict__ prs, const uint32_t* __restrict__ keys,
+                                              int64_t n, int64_t nblk, float* __restrict__ dtot) {
   constexpr int SYM = (NB - 1) / 2;
   const int W = 2 * d.nn + 1;
   const int64_t units = (int64_t)W * W * nblk;
@@ -554,13 +554,14 @@ __global__ __launch_bounds__(256) void k_corr(Dev d, Tables tb, const SegRec* __
   if (s >= n || keys[s] >= (uint32_t)d.n_cells) return;
   const int a = ab / W - d.nn, b = ab % W - d.nn;
   const int nsb = d.nsb, cm = d.cm;
-  const FarRec F = *reinterpret_cast<const FarRec*>(rec + s);
-  const int ux = F.uxy & 0xffff, uy = F.uxy >> 16;
+  const PairRec F = prs[s];  // 32 bytes per segment: the whole pass re-reads 32 B, not 80 B, per offset
+  const int ux = (F.flags >> 8) & 0xff, uy = (F.flags >> 16) & 0xff;
+  const int k0 = d.nt - d.L - F.S;
   const float f = F.f;
   float delta = 0.f;
   if (a != 0 || b != 0) {
     const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
-    const float2 D = tb.DN[sidx * d.nt + F.k0];
+    const float2 D = tb.DN[sidx * d.nt + k0];
     delta = F.ne * (D.x - f * D.y);
   }
   if (a >= -1 && a <= 1 && b >= -1 && b <= 1) {
@@ -579,7 +580,7 @@ __global__ __launch_bounds__(256) void k_corr(Dev d, Tables tb, const SegRec* __
         if (dy != b) continue;
         const float c = wxs * R.wy[jj];
         if (c != 0.f) {
-          const float2 D = tb.DM[(((int64_t)R.idx * cm + cx) * cm + offset_bin(vy - dy * nsb, 0, nsb)) * d.nt + F.k0];
+          const float2 D = tb.DM[(((int64_t)R.idx * cm + cx) * cm + offset_bin(vy - dy * nsb, 0, nsb)) * d.nt + k0];
           delta += c * (D.x - f * D.y);
         }
       }
@@ -745,7 +746,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
       for (int g = 0; g < kFarGroup; ++g)
         if (P[g].ne != 0.f)
           store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, 1.f, 1.f - P[g].f, P[g].f, q[g].D, P[g].S, d.L, d.nout,
-                       P[g].interior != 0);
+                       (P[g].flags & 1) != 0);
     }
   }
   {

@@ -469,13 +469,13 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_s
     const unsigned gc = (unsigned)(((units + kXcd - 1) / kXcd) * kXcd);
     if (d.nb == 5) {
       hipLaunchKernelGGL(k_center<5>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-      hipLaunchKernelGGL(k_corr<5>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, nblk, w.dtot);
+      hipLaunchKernelGGL(k_corr<5>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, w.prs, keys, n_seg, nblk, w.dtot);
     } else if (d.nb == 3) {
       hipLaunchKernelGGL(k_center<3>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-      hipLaunchKernelGGL(k_corr<3>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, nblk, w.dtot);
+      hipLaunchKernelGGL(k_corr<3>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, w.prs, keys, n_seg, nblk, w.dtot);
     } else {
       hipLaunchKernelGGL(k_center<1>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-      hipLaunchKernelGGL(k_corr<1>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, nblk, w.dtot);
+      hipLaunchKernelGGL(k_corr<1>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, w.prs, keys, n_seg, nblk, w.dtot);
     }
   }
   if (max_rows > 0) {
@@ -669,7 +669,8 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
       hipLaunchKernelGGL((k_rows_adj<1, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell, w.n_rows,
                          max_rows, pieces, partial);
   }
-  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(256), 0, st, d, partial, max_rows, dz, fwd->n_rows, max_rows, d_params);
+  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(kReduceThreads), 0, st, d, partial, max_rows, dz, fwd->n_rows, max_rows,
+                     d_params);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }
