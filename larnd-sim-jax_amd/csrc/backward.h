@@ -154,30 +154,94 @@ __global__ __launch_bounds__(256) void k_adj_records(Dev d, SegIn in, const uint
   adj[s] = A;
 }
 
-// The window dot products of one (segment, row) pair against the row's step function G held in
-// LDS (pieces px/pc, np of them): y1 = sum_{o=lo1..hi1} G(o) X[o-S], y0 = sum_{o=lo0..hi0} G(o)
-// X[o-S+1], from the window prefix sums CX (CX[k] = sum_{j<k} X[j]); NT tables at stride ts.
+// The row's step function G in LDS: breakpoints x[q] (ascending) and values c[q], G(o) = c[q] on
+// (x[q-1], x[q]] and 0 past x[np-1]; jumps dc[q] = c[q] - c[q+1] (c[np] = 0); and qlut[o] = the first
+// q with x[q] >= o, for 0 <= o < nout (np when none).
+struct RowG {
+  const int* x;
+  const float* c;
+  const float* dc;
+  const uint8_t* qlut;
+  int np;
+  __device__ __forceinline__ double at(int o) const {  // G(o), 0 <= o < nout
+    const int q = qlut[o];
+    return q < np ? (double)c[q] : 0.0;
+  }
+};
+
+// Window dot products of one (segment, row) pair, telescoped over G's breakpoints.  With
+// CX[k] = sum_{j<k} X[j] and F(o) = CX[o - S + sh + 1],
+//   sum_{o=lo}^{hi} G(o) X[o - S + sh] = G(hi) F(hi) - G(lo) F(lo - 1) + sum_{lo <= x_q < hi} dc_q F(x_q),
+// so each breakpoint inside the window costs one prefix-sum load per table.  y1: the (1-f) window
+// (sh = 0, o in [lo1, hi1]); y0: the f window (sh = 1, o in [lo0, hi0]).  NT tables at stride ts.
 template <int NT>
-__device__ __forceinline__ void g_dots(const int* __restrict__ px, const float* __restrict__ pc, int np, int q0,
-                                       const double* __restrict__ CX, int64_t ts, int S, int lo1, int hi1, int lo0,
-                                       int hi0, double* y1, double* y0) {
+__device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__ CX, int64_t ts, int S, int lo1, int hi1,
+                                       int lo0, int hi0, double* y1, double* y0) {
 #pragma unroll
   for (int k = 0; k < NT; ++k) y1[k] = y0[k] = 0.0;
-  int prev = q0 > 0 ? px[q0 - 1] : -1;  // piece q covers (prev, x_q]
-  for (int q = q0; q < np && prev < hi1; ++q) {
-    const int xq = px[q];
-    const double c = (double)pc[q];
-    const int a1 = (prev + 1) > lo1 ? (prev + 1) : lo1, b1 = xq < hi1 ? xq : hi1;
-    const int a0 = (prev + 1) > lo0 ? (prev + 1) : lo0, b0 = xq < hi0 ? xq : hi0;
-    if (a1 <= b1) {
+  const bool w1 = lo1 <= hi1, w0 = lo0 <= hi0;
+  if (w1) {
+    const double gh = G.at(hi1), gl = G.at(lo1);
 #pragma unroll
-      for (int k = 0; k < NT; ++k) y1[k] += c * (CX[k * ts + b1 - S + 1] - CX[k * ts + a1 - S]);
-    }
-    if (a0 <= b0) {
+    for (int k = 0; k < NT; ++k) y1[k] = gh * CX[k * ts + hi1 - S + 1] - gl * CX[k * ts + lo1 - S];
+  }
+  if (w0) {
+    const double gh = G.at(hi0), gl = G.at(lo0);
 #pragma unroll
-      for (int k = 0; k < NT; ++k) y0[k] += c * (CX[k * ts + b0 - S + 2] - CX[k * ts + a0 - S + 1]);
+    for (int k = 0; k < NT; ++k) y0[k] = gh * CX[k * ts + hi0 - S + 2] - gl * CX[k * ts + lo0 - S + 1];
+  }
+  if (!w1 && !w0) return;
+  const int lo = lo0 < lo1 ? lo0 : lo1;
+  for (int q = G.qlut[lo]; q < G.np; ++q) {
+    const int xq = G.x[q];
+    if (xq >= hi1) break;  // hi0 <= hi1
+    const double dq = (double)G.dc[q];
+    if (w1 && xq >= lo1) {
+#pragma unroll
+      for (int k = 0; k < NT; ++k) y1[k] += dq * CX[k * ts + xq - S + 1];
     }
-    prev = xq;
+    if (w0 && xq >= lo0 && xq < hi0) {
+#pragma unroll
+      for (int k = 0; k < NT; ++k) y0[k] += dq * CX[k * ts + xq - S + 2];
+    }
+  }
+}
+
+// Distinct LUT cells of the diffusion bins that land on the row along one axis (pixel offset a):
+// bins with the same cell have their weights summed (the bin weights factorise, w = wx wy, and
+// every other factor depends on the cell only), so each distinct (cx, cy) is visited once.
+template <int NB>
+__device__ __forceinline__ void axis_cells(int u, int a, int nsb, const float* w, const float* dw, bool* ok, int* cell,
+                                           float* W, float* DW) {
+  constexpr int SYM = (NB - 1) / 2;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    ok[i] = false;
+    cell[i] = 0;
+    W[i] = 0.f;
+    DW[i] = 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int v = u + i - SYM;
+    const int dv = v < 0 ? -1 : (v >= nsb ? 1 : 0);
+    if (dv != a) continue;
+    const int cc = offset_bin(v - dv * nsb, 0, nsb);
+    bool merged = false;
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      if (!merged && ok[j] && cell[j] == cc) {
+        W[j] += w[i];
+        DW[j] += dw[i];
+        merged = true;
+      }
+    }
+    if (!merged) {
+      ok[i] = true;
+      cell[i] = cc;
+      W[i] = w[i];
+      DW[i] = dw[i];
+    }
   }
 }
 
@@ -185,25 +249,16 @@ __device__ __forceinline__ void g_dots(const int* __restrict__ px, const float* 
 // (main pixel + (a, b)): the neighbour stripe unless (a, b) = (0, 0), and the diffusion bins
 // landing on the row when |a|, |b| <= 1.  Forward placement (sim_jax.py:170-261): (1-f) V at
 // o = S + j, f V at o = S - 1 + j, (1-f) Delta at S - 1 and f Delta at S - 2, S = nt - L - k0.
-template <int NB>
+template <int NB, bool NEAR>
 __device__ __forceinline__ void pair_adjoint(const Dev& d, const Tables& tb, const SegRec& R,
-                                             const AdjRec* __restrict__ Ap, int a, int b,
-                                             const int* __restrict__ px, const float* __restrict__ pc, int np,
-                                             double* g) {
-  constexpr int SYM = (NB - 1) / 2;
+                                             const AdjRec* __restrict__ Ap, int a, int b, const RowG& G, double* g) {
   const int L = d.L, nt = d.nt, nout = d.nout, nsb = d.nsb;
   const int S = nt - L - R.k0;
-  int q0 = 0;
-  while (q0 < np && px[q0] < S - 2) ++q0;
-  if (q0 == np) return;  // G vanishes from tick S - 2 on
+  const int o2 = S - 2 > 0 ? S - 2 : 0;
+  if (G.qlut[o2] >= G.np) return;  // G vanishes from tick S - 2 on
   const double f = (double)R.f, ne = (double)R.ne;
-  // G(S - 2) = c[q0]; G(S - 1) is c[q0] unless piece q0 ends at S - 2
-  const double gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? (double)pc[q0] : 0.0;
-  double gc0 = 0.0;
-  if (S - 1 >= 0 && S - 1 <= nout - 2) {
-    const int q = px[q0] >= S - 1 ? q0 : q0 + 1;
-    gc0 = q < np ? (double)pc[q] : 0.0;
-  }
+  const double gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? G.at(S - 2) : 0.0;
+  const double gc0 = (S - 1 >= 0 && S - 1 <= nout - 2) ? G.at(S - 1) : 0.0;
   const double gc = (1.0 - f) * gc0 + f * gc1;
   const int lo1 = S > 1 ? S : 1, hi1 = (S + L - 1) < (nout - 1) ? (S + L - 1) : (nout - 1);
   const int lo0 = (S - 1) > 1 ? (S - 1) : 1, hi0 = (S + L - 2) < (nout - 1) ? (S + L - 2) : (nout - 1);
@@ -212,44 +267,41 @@ __device__ __forceinline__ void pair_adjoint(const Dev& d, const Tables& tb, con
   if (a != 0 || b != 0) {
     const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
     double y1, y0;
-    g_dots<1>(px, pc, np, q0, tb.CTN + sidx * (L + 1), 0, S, lo1, hi1, lo0, hi0, &y1, &y0);
+    g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, S, lo1, hi1, lo0, hi0, &y1, &y0);
     const float2 D = tb.DN[sidx * nt + R.k0];
     const double corr = (double)D.x - f * (double)D.y;
     gN += (1.0 - f) * y1 + f * y0 + gc * corr;
     gF += ne * (y0 - y1 + (gc1 - gc0) * corr - gc * (double)D.y);
   }
-  bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
-  if (NB == 1 && !(a == 0 && b == 0)) near = false;
+  constexpr bool near = NEAR;  // the caller passes the 3x3 near cells only to the NEAR instantiation
   if (near) {
     const int cm = d.cm;
     const int64_t tstride = (int64_t)cm * cm * (L + 1);
     const double la = (double)R.la, lb = (double)R.lb, lc = (double)R.lc;
     const double da = Ap->dl[0], db = Ap->dl[1], dc = Ap->dl[2];
+    bool okx[NB], oky[NB];
+    int cxv[NB], cyv[NB];
+    float WX[NB], DWX[NB], WY[NB], DWY[NB];
+    axis_cells<NB>(ux, a, nsb, R.wx, Ap->dwx, okx, cxv, WX, DWX);
+    axis_cells<NB>(uy, b, nsb, R.wy, Ap->dwy, oky, cyv, WY, DWY);
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      const int vx = ux + i - SYM;
-      const int dxp = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
-      if (dxp != a) continue;
-      const int cx = offset_bin(vx - dxp * nsb, 0, nsb);
-      const double wxi = (double)R.wx[i], dwxi = (double)Ap->dwx[i];
+      if (!okx[i]) continue;
 #pragma unroll
       for (int jj = 0; jj < NB; ++jj) {
-        const int vy = uy + jj - SYM;
-        const int dyp = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
-        if (dyp != b) continue;
-        const int cy = offset_bin(vy - dyp * nsb, 0, nsb);
-        const double wyj = (double)R.wy[jj];
-        const double* C1 = tb.CTM + (((int64_t)(R.idx - 1) * cm + cx) * cm + cy) * (L + 1);
+        if (!oky[jj]) continue;
+        const int64_t cell = ((int64_t)R.idx * cm + cxv[i]) * cm + cyv[jj];
         double y1[3], y0[3];
-        g_dots<3>(px, pc, np, q0, C1, tstride, S, lo1, hi1, lo0, hi0, y1, y0);
-        const float2 D = tb.DM[(((int64_t)R.idx * cm + cx) * cm + cy) * nt + R.k0];
+        g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, S, lo1, hi1, lo0, hi0, y1, y0);
+        const float2 D = tb.DM[cell * nt + R.k0];
         const double corr = (double)D.x - f * (double)D.y;
-        const double wxy = wxi * wyj;
+        const double wxy = (double)WX[i] * (double)WY[jj];
+        const double dwxy = (double)DWX[i] * (double)WY[jj] + (double)WX[i] * (double)DWY[jj];
         const double Y1 = la * y1[0] + lb * y1[1] + lc * y1[2];
         const double Y0 = la * y0[0] + lb * y0[1] + lc * y0[2];
-        const double gn = (1.0 - f) * Y1 + f * Y0 + gc * corr;  // d loss / d n_bin
+        const double gn = (1.0 - f) * Y1 + f * Y0 + gc * corr;  // d loss / d n_bin per unit weight
         gN += gn * wxy;
-        gS += gn * ne * (dwxi * wyj + wxi * (double)Ap->dwy[jj]);
+        gS += gn * ne * dwxy;
         gF += ne * wxy * (Y0 - Y1 + (gc1 - gc0) * corr - gc * (double)D.y);
         gL += ne * wxy * ((1.0 - f) * (da * y1[0] + db * y1[1] + dc * y1[2]) +
                           f * (da * y0[0] + db * y0[1] + dc * y0[2]));
@@ -268,10 +320,12 @@ __device__ __forceinline__ void pair_adjoint(const Dev& d, const Tables& tb, con
 }
 
 // Stage 2, row-owned: one workgroup per pixel row holds the row's step function in LDS and walks
-// the segments of the (2n+1)^2 neighbourhood cells (the same pairs the forward's waveform kernel
-// visits), every thread taking pairs from the flattened list; fixed-order block reduction to the
-// row's 7 partial gradients.  Rows without gradient (no valid hit) exit at once.
-template <int NB, int WAVES>
+// the segments of its neighbourhood cells (the same pairs the forward's waveform kernel visits),
+// every thread taking pairs from the flattened list; fixed-order block reduction to the row's 7
+// partial gradients.  Two passes (separate launches, separate partials): NEAR = false takes the
+// neighbour-only cells (|a| or |b| > 1; little register state, high occupancy), NEAR = true the
+// 3x3 cells with the diffusion bins.  Rows without gradient (no valid hit) exit at once.
+template <int NB, int WAVES, bool NEAR>
 __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const AdjRec* __restrict__ adj, const int2* __restrict__ rng,
                                                             const int* __restrict__ row_cell,
@@ -282,7 +336,8 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
   __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
   __shared__ int n_cells_tot;
   __shared__ int spx[2 * kMaxH];
-  __shared__ float spc[2 * kMaxH];
+  __shared__ float spc[2 * kMaxH], sdc[2 * kMaxH];
+  __shared__ uint8_t qlut[kMaxOut];
   __shared__ double red[kNGrad][NT / kWave];
   const int t = threadIdx.x;
   const int P0 = *n_rows_dev;
@@ -304,6 +359,14 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
   if (t < np) {
     spx[t] = pr->x[t];
     spc[t] = pr->c[t];
+    sdc[t] = pr->c[t] - (t + 1 < np ? pr->c[t + 1] : 0.f);
+  }
+  __syncthreads();
+  // qlut[o] = number of breakpoints below o = first q with x[q] >= o
+  for (int o = t; o < d.nout; o += NT) {
+    int q = 0;
+    for (int k = 0; k < np; ++k) q += spx[k] < o ? 1 : 0;
+    qlut[o] = (uint8_t)q;
   }
   if (t == 0) {
     const int cell = row_cell[r];
@@ -312,11 +375,11 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
     const int evp = cell / (d.gx * d.gy);
     int k = 0, cum = 0;
     c_cum[0] = 0;
-    for (int pass = 0; pass < 2; ++pass) {  // neighbour-only cells first, then the 3x3 near cells
+    {
       for (int a = -d.nn; a <= d.nn; ++a) {
         for (int b = -d.nn; b <= d.nn; ++b) {
           const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
-          if ((int)near != pass) continue;
+          if (near != NEAR) continue;
           const int qx = px - a, qy = py - b;
           if (qx < -d.halo || qx >= d.nx + d.halo || qy < -d.halo || qy >= d.ny + d.halo) continue;
           const int2 rg = rng[((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo)];
@@ -335,6 +398,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
   double g[kNGrad];
 #pragma unroll
   for (int k = 0; k < kNGrad; ++k) g[k] = 0.0;
+  const RowG G{spx, spc, sdc, qlut, np};
   const int nct = n_cells_tot;
   const int tot = c_cum[nct];
   int cur = 0;
@@ -344,7 +408,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
     const int ab = c_ab[cur];
     const SegRec R = rec[s];
     if (R.ne == 0.f) continue;
-    pair_adjoint<NB>(d, tb, R, adj + s, (ab >> 8) - 64, (ab & 255) - 64, spx, spc, np, g);
+    pair_adjoint<NB, NEAR>(d, tb, R, adj + s, (ab >> 8) - 64, (ab & 255) - 64, G, g);
   }
   const int lane = t & (kWave - 1), wid = t / kWave;
 #pragma unroll

@@ -619,7 +619,7 @@ int lsim_backward_query(lsim_ctx* c, int64_t n_seg, int64_t max_rows, size_t* by
   Carve cv(nullptr);
   cv.take<Pieces>(max_rows);
   cv.take<double>(max_rows);
-  cv.take<double>(max_rows * kNGrad);
+  cv.take<double>(2 * max_rows * kNGrad);
   cv.take<AdjRec>(n_seg);
   *bytes = cv.off + 256;
   return LSIM_OK;
@@ -647,7 +647,7 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
   Carve cv(bwd_ws);
   Pieces* pieces = cv.take<Pieces>(max_rows);
   double* dz = cv.take<double>(max_rows);
-  double* partial = cv.take<double>(max_rows * kNGrad);
+  double* partial = cv.take<double>(2 * max_rows * kNGrad);  // far pass rows, then near pass rows
   AdjRec* adj = cv.take<AdjRec>(n);
   Tables tb{c->TM3, c->TM, c->TN, c->TNP, c->DM, c->DN, c->CTM, c->CTN};
   SegIn in = make_segin(seg);
@@ -659,18 +659,23 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
     if (n > 0)
       hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, in, keys, vals, n, adj);
     const dim3 grid((unsigned)max_rows), block(kAdjWaves * kWave);
-    if (d.nb == 5)
-      hipLaunchKernelGGL((k_rows_adj<5, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell, w.n_rows,
-                         max_rows, pieces, partial);
-    else if (d.nb == 3)
-      hipLaunchKernelGGL((k_rows_adj<3, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell, w.n_rows,
-                         max_rows, pieces, partial);
-    else
-      hipLaunchKernelGGL((k_rows_adj<1, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell, w.n_rows,
-                         max_rows, pieces, partial);
+    double* pnear = partial + max_rows * kNGrad;
+#define LSIM_ADJ_LAUNCH(NBV)                                                                                        \
+  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell,  \
+                     w.n_rows, max_rows, pieces, partial);                                                          \
+  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell,   \
+                     w.n_rows, max_rows, pieces, pnear)
+    if (d.nb == 5) {
+      LSIM_ADJ_LAUNCH(5);
+    } else if (d.nb == 3) {
+      LSIM_ADJ_LAUNCH(3);
+    } else {
+      LSIM_ADJ_LAUNCH(1);
+    }
+#undef LSIM_ADJ_LAUNCH
   }
-  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(kReduceThreads), 0, st, d, partial, max_rows, dz, fwd->n_rows, max_rows,
-                     d_params);
+  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(kReduceThreads), 0, st, d, partial, 2 * max_rows, dz, fwd->n_rows,
+                     max_rows, d_params);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }
