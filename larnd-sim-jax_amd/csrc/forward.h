@@ -407,6 +407,43 @@ __device__ __forceinline__ void far_store(const Dev& d, const PairRec& P, const 
   store_window(accw, lane, q.t0, q.t1, q.t2, P.ne, 1.f - P.f, P.f, q.D, P.S, d.L, d.nout, (P.flags & 1) != 0);
 }
 
+// Distinct LUT cells of a segment's diffusion bins that land on pixel offset a along one axis, with
+// the bins' weights summed per cell: bins on the same cell read the same template window and the
+// bin weights factorise (n_bin = n_e wx wy), so each distinct (cx, cy) is visited once with weight
+// W[i] W[j].  ok[i] marks the distinct entries (slot of the first bin of each cell).
+template <int NB>
+__device__ __forceinline__ void axis_merge(int u, int a, int nsb, const float* w, float scale, bool* ok, int* cell,
+                                           float* W) {
+  constexpr int SYM = (NB - 1) / 2;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    ok[i] = false;
+    cell[i] = 0;
+    W[i] = 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int v = u + i - SYM;
+    const int dv = v < 0 ? -1 : (v >= nsb ? 1 : 0);
+    if (dv != a) continue;
+    const int cc = offset_bin(v - dv * nsb, 0, nsb);
+    const float wi = scale * w[i];
+    bool merged = false;
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      if (!merged && ok[j] && cell[j] == cc) {
+        W[j] += wi;
+        merged = true;
+      }
+    }
+    if (!merged) {
+      ok[i] = true;
+      cell[i] = cc;
+      W[i] = wi;
+    }
+  }
+}
+
 // |a|, |b| <= 1: the diffusion bins landing on this row (interpolated templates) plus the
 // neighbour stripe unless (a, b) = (0, 0).  TM3[idx][cx][cy][j] holds (T_{idx-1}, T_idx,
 // T_{idx+1}) so one 12-byte load serves the three Lagrange nodes.  Bin i of the segment lies
@@ -484,50 +521,40 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
 template <int NB>
 __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                 const uint32_t* __restrict__ keys, int64_t n, float* __restrict__ vc) {
-  constexpr int SYM = (NB - 1) / 2;
   const int lane = threadIdx.x & (kWave - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
   if (s >= n || keys[s] >= (uint32_t)d.n_cells) return;
   const SegRec R = rec[s];
-  const int nsb = d.nsb, L = d.L, cm = d.cm;
+  const int nsb = d.nsb, cm = d.cm;
   const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
+  bool okx[NB], oky[NB];
   int cxs[NB], cys[NB];
-  float wxs[NB], wys[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) {
-    int vx = ux + i - SYM, vy = uy + i - SYM;
-    const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
-    const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
-    cxs[i] = offset_bin(vx - dx * nsb, 0, nsb);
-    cys[i] = offset_bin(vy - dy * nsb, 0, nsb);
-    wxs[i] = (dx == 0) ? R.ne * R.wx[i] : 0.f;
-    wys[i] = (dy == 0) ? R.wy[i] : 0.f;
-  }
-  const float f = R.f, la = R.la, lb = R.lb, lc = R.lc;
-  const int idx = R.idx, k0 = R.k0;
+  float WX[NB], WY[NB];
+  axis_merge<NB>(ux, 0, nsb, R.wx, R.ne, okx, cxs, WX);
+  axis_merge<NB>(uy, 0, nsb, R.wy, 1.f, oky, cys, WY);
+  const float la = R.la, lb = R.lb, lc = R.lc;
+  const int idx = R.idx;
   const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * kWinPad + lane;
   // padded positions p = lane, lane + 64, lane + 128 (kWinPad = 3 * 64)
   float v1 = 0.f, v2 = 0.f, v3 = 0.f;
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    if (wxs[i] != 0.f) {
+    if (!okx[i]) continue;
 #pragma unroll
-      for (int jj = 0; jj < NB; ++jj) {
-        const float c = wxs[i] * wys[jj];
-        const int coff = cxs[i] * cm + cys[jj];
-        const float3* tp = T3 + (int64_t)coff * kWinPad;
-        const float3 q1 = tp[0], q2 = tp[kWave], q3 = tp[2 * kWave];
-        v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
-        v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
-        v3 += c * (q3.y * lb + q3.x * la + q3.z * lc);
-      }
+    for (int jj = 0; jj < NB; ++jj) {
+      if (!oky[jj]) continue;
+      const float c = WX[i] * WY[jj];
+      const float3* tp = T3 + ((int64_t)cxs[i] * cm + cys[jj]) * kWinPad;
+      const float3 q1 = tp[0], q2 = tp[kWave], q3 = tp[2 * kWave];
+      v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
+      v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
+      v3 += c * (q3.y * lb + q3.x * la + q3.z * lc);
     }
   }
   float* o = vc + (int64_t)s * kWinPad + lane;
   o[0] = v1;
   o[kWave] = v2;
   o[2 * kWave] = v3;
-  (void)L;
 }
 
 // Boundary corrections of every (segment, row offset) pair, one thread per pair:
