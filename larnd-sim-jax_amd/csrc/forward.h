@@ -740,15 +740,19 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         const int jj = j < cnt ? j : k;
         P[g] = pr[jj];
         D[g] = dt[jj];
-        if (j >= cnt) P[g].ne = 0.f;
+        if (j >= cnt) {  // padding slot of the last group: adds exactly zero
+          P[g].ne = 0.f;
+          D[g] = 0.f;
+        }
       }
       FarLoads q[kFarGroup];
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g)
         far_load(tnp, ma * P[g].U + za * P[g].U0 + mb * P[g].V + zb * P[g].V0, D[g], q[g]);
+      // unconditional stores: a branch on the charge would let the compiler sink a pair's loads
+      // into it and wait on them there instead of overlapping them with the other pairs'
 #pragma unroll
-      for (int g = 0; g < kFarGroup; ++g)
-        if (P[g].ne != 0.f) far_store(d, P[g], q[g], accw, lane);
+      for (int g = 0; g < kFarGroup; ++g) far_store(d, P[g], q[g], accw, lane);
     }
   }
   // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
@@ -759,21 +763,22 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
       PairRec P[kFarGroup];
       int sg[kFarGroup];
+      float sc[kFarGroup];
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g) {
         const int j = k + g * WAVES;
         sg[g] = beg + (j < cnt ? j : k);
         P[g] = prs[sg[g]];
-        if (j >= cnt) P[g].ne = 0.f;
+        sc[g] = j < cnt ? 1.f : 0.f;  // padding slot of the last group: adds exactly zero
       }
       FarLoads q[kFarGroup];
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g) center_load(d, vc, dtc, sg[g], lane, q[g]);
+      // unconditional stores (see the far pairs)
 #pragma unroll
       for (int g = 0; g < kFarGroup; ++g)
-        if (P[g].ne != 0.f)
-          store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, 1.f, 1.f - P[g].f, P[g].f, q[g].D, P[g].S, d.L, d.nout,
-                       (P[g].flags & 1) != 0);
+        store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, sc[g], 1.f - P[g].f, P[g].f, sc[g] * q[g].D, P[g].S, d.L,
+                     d.nout, (P[g].flags & 1) != 0);
     }
   }
   {
