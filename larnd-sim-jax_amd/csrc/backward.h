@@ -334,7 +334,8 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
                                                             double* __restrict__ partial) {
   constexpr int NT = WAVES * kWave;
   __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
-  __shared__ int n_cells_tot;
+  __shared__ int ncls[3];
+  __shared__ int cell_scratch[3 * kMaxCells];
   __shared__ int spx[2 * kMaxH];
   __shared__ float spc[2 * kMaxH], sdc[2 * kMaxH];
   __shared__ uint8_t qlut[kMaxOut];
@@ -368,38 +369,19 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
     for (int k = 0; k < np; ++k) q += spx[k] < o ? 1 : 0;
     qlut[o] = (uint8_t)q;
   }
-  if (t == 0) {
-    const int cell = row_cell[r];
-    const int px = cell % d.gx - d.halo;
-    const int py = (cell / d.gx) % d.gy - d.halo;
-    const int evp = cell / (d.gx * d.gy);
-    int k = 0, cum = 0;
-    c_cum[0] = 0;
-    {
-      for (int a = -d.nn; a <= d.nn; ++a) {
-        for (int b = -d.nn; b <= d.nn; ++b) {
-          const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
-          if (near != NEAR) continue;
-          const int qx = px - a, qy = py - b;
-          if (qx < -d.halo || qx >= d.nx + d.halo || qy < -d.halo || qy >= d.ny + d.halo) continue;
-          const int2 rg = rng[((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo)];
-          if (rg.y <= rg.x) continue;
-          c_beg[k] = rg.x;
-          cum += rg.y - rg.x;
-          c_cum[k + 1] = cum;
-          c_ab[k] = ((a + 64) << 8) | (b + 64);
-          ++k;
-        }
-      }
-    }
-    n_cells_tot = k;
-  }
-  __syncthreads();
+  // this pass's cells in (a, b) order
+  list_cells<NT>(
+      d, rng, row_cell[r],
+      [](int a, int b) {
+        const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
+        return near == NEAR ? 0 : -1;
+      },
+      cell_scratch, c_beg, c_cum, c_ab, ncls);
   double g[kNGrad];
 #pragma unroll
   for (int k = 0; k < kNGrad; ++k) g[k] = 0.0;
   const RowG G{spx, spc, sdc, qlut, np};
-  const int nct = n_cells_tot;
+  const int nct = ncls[2];
   const int tot = c_cum[nct];
   int cur = 0;
   for (int i = t; i < tot; i += NT) {

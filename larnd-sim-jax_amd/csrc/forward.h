@@ -201,16 +201,19 @@ __device__ __forceinline__ int wave_min_i(int v) {
 }
 
 // Self-trigger of one row (fee_jax.py:168-279, noise off) on Q held in LDS, then digitisation
-// (fee_jax.py:55-71).  Q and Q0 hold the float64 cumulative charge; Q is consumed.
+// (fee_jax.py:55-71).  Q and Q0 hold the float64 cumulative charge; Q is consumed.  The passes'
+// decisions go to LDS; wave 0 then writes the outputs and the tape with one lane per pass (and
+// one lane per tape point for the clamp masks).
 template <int NT>
 __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int plane, const RowOut& out, int* sh_int) {
   constexpr int TPER = kMaxOut / NT;
+  __shared__ int s_pe[kMaxH], s_pg[kMaxH], s_pi[kMaxH];
+  __shared__ double s_pa[kMaxH];
   const int t = threadIdx.x;
   const int nout = d.nout;
   const int lane = t & (kWave - 1), wid = t / kWave;
-  int pe[kMaxH], pg[kMaxH], pi[kMaxH];
-  double pa[kMaxH];
-  for (int k = 0; k < d.H; ++k) {
+  const int H = d.H;
+  for (int k = 0; k < H; ++k) {
     int cand = 0x7fffffff;
 #pragma unroll
     for (int u = 0; u < TPER; ++u) {
@@ -229,12 +232,13 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
     if (e >= nout) e = nout - 1;
     int g = istar + 1 + d.interval + 1;
     if (g >= nout) g = nout - 1;
-    const double a = Q[e];
     const double Qg = Q[g];
-    pe[k] = e;
-    pg[k] = g;
-    pi[k] = istar;
-    pa[k] = a;
+    if (t == 0) {
+      s_pe[k] = e;
+      s_pg[k] = g;
+      s_pi[k] = istar;
+      s_pa[k] = Q[e];
+    }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < TPER; ++u) {
@@ -246,44 +250,46 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
     }
     __syncthreads();
   }
-  if (t == 0) {
-    int hits = 0;
-    const int H = d.H;
-    for (int k = 0; k < H; ++k) {
-      const bool invalid = (pa[k] < d.thr) || (pi[k] == nout - 2);
-      const double q = invalid ? 0.0 : pa[k];
-      const int tick = invalid ? nout - 2 : pi[k];
-      out.adc[(int64_t)r * H + k] = (float)digitize(d, q);
-      out.ticks[(int64_t)r * H + k] = tick;
-      hits += (tick < nout - 3) ? 1 : 0;
-    }
-    out.hit_cnt[r] = hits;
-    if (out.tape) {
-      FeeTape tp;
-      double val[2 * kMaxH];
-      for (int k = 0; k < H; ++k) {
-        tp.pt[k] = (int16_t)pe[k];
-        tp.pt[H + k] = (int16_t)pg[k];
-        val[k] = Q0[pe[k]];
-        val[H + k] = Q0[pg[k]];
-      }
-      for (int k = 0; k < H; ++k) {
-        const bool invalid = (pa[k] < d.thr) || (pi[k] == nout - 2);
-        tp.slope[k] = invalid ? 0.f : (float)digitize_slope(d, pa[k]);
-        tp.tick[k] = invalid ? nout - 2 : pi[k];
-        const double qg = val[H + k];
-        uint32_t m = 0;
-        for (int j = 0; j < 2 * H; ++j) {
-          const double x = val[j] - qg;
-          if (x >= 0) m |= (1u << j);
-          val[j] = x < 0 ? 0.0 : x;
-        }
-        tp.mask[k] = m;
-      }
-      tp.plane = plane;
-      out.tape[r] = tp;
-    }
+  if (t >= kWave) return;
+  // lane k < H: pass k's hit
+  bool hit = false;
+  float slope = 0.f;
+  int tick = nout - 2;
+  if (t < H) {
+    const double a = s_pa[t];
+    const int i = s_pi[t];
+    const bool invalid = (a < d.thr) || (i == nout - 2);
+    tick = invalid ? nout - 2 : i;
+    out.adc[(int64_t)r * H + t] = (float)digitize(d, invalid ? 0.0 : a);
+    out.ticks[(int64_t)r * H + t] = tick;
+    hit = tick < nout - 3;
+    slope = invalid ? 0.f : (float)digitize_slope(d, a);
   }
+  const int hits = __popcll(__ballot(hit));
+  if (t == 0) out.hit_cnt[r] = hits;
+  if (!out.tape) return;
+  // the clamp masks: lane j < 2H follows Q at point j (e_j, then g_{j-H}) through the passes
+  FeeTape* tp = out.tape + r;
+  double v = 0.0;
+  if (t < 2 * H) {
+    const int p = t < H ? s_pe[t] : s_pg[t - H];
+    tp->pt[t] = (int16_t)p;
+    v = Q0[p];
+  }
+  uint32_t mine = 0;
+  for (int k = 0; k < H; ++k) {
+    const double qg = __shfl(v, H + k, kWave);
+    const double x = v - qg;
+    const uint32_t m = (uint32_t)(__ballot(t < 2 * H && x >= 0) & 0xffffffffull);
+    if (t == k) mine = m;
+    v = x < 0 ? 0.0 : x;
+  }
+  if (t < H) {
+    tp->mask[t] = mine;
+    tp->slope[t] = slope;
+    tp->tick[t] = tick;
+  }
+  if (t == 0) tp->plane = plane;
 }
 
 // Sum of the per-wave partial rows, waveform store, float64 cumulative charge into Q and Q0.
@@ -635,6 +641,78 @@ inline size_t rows_lds_bytes(int waves, int nout) {
 }
 
 constexpr int kMaxCells = 17 * 17;  // (2n+1)^2 for number_pix_neighbors <= 8
+
+// The non-empty cells of a row's neighbourhood.  Cell i = (a, b), a, b in [-nn, nn] in row-major
+// order, is the segment cell of the pixel (px - a, py - b); it is listed when cls(a, b) >= 0 and
+// it holds segments, ordered by (cls, i).  One thread per cell loads the cell's segment range;
+// each listed cell then finds its place and the number of segments before it by one pass over
+// the keys in LDS.  Writes c_beg, c_ab (a + 64 << 8 | b + 64), c_cum[0..k] (exclusive sums of the
+// cells' segment counts) and ncls[c] = number of listed cells of class <= c, c < 3.  scratch
+// holds 3 * kMaxCells ints; the call begins and ends with every thread of the block in it.
+template <int NT, class Cls>
+__device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict__ rng, int cell, Cls cls, int* scratch,
+                                           int* c_beg, int* c_cum, int* c_ab, int* ncls) {
+  const int t = threadIdx.x;
+  const int side = 2 * d.nn + 1, ncell = side * side;
+  const int px = cell % d.gx - d.halo;
+  const int py = (cell / d.gx) % d.gy - d.halo;
+  const int evp = cell / (d.gx * d.gy);
+  int* s_key = scratch;
+  int* s_cnt = scratch + kMaxCells;
+  int* s_beg = scratch + 2 * kMaxCells;
+  for (int i = t; i < ncell; i += NT) {
+    const int a = i / side - d.nn, b = i % side - d.nn;
+    const int c = cls(a, b);
+    const int qx = px - a, qy = py - b;
+    int key = -1, n = 0, beg = 0;
+    if (c >= 0 && qx >= -d.halo && qx < d.nx + d.halo && qy >= -d.halo && qy < d.ny + d.halo) {
+      const int2 rg = rng[((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo)];
+      if (rg.y > rg.x) {
+        key = c * ncell + i;
+        n = rg.y - rg.x;
+        beg = rg.x;
+      }
+    }
+    s_key[i] = key;
+    s_cnt[i] = n;
+    s_beg[i] = beg;
+  }
+  __syncthreads();
+  for (int i = t; i < ncell; i += NT) {
+    const int key = s_key[i];
+    if (key < 0) continue;
+    int rank = 0, before = 0;
+    for (int j = 0; j < ncell; ++j) {
+      const int kj = s_key[j];
+      if (kj >= 0 && kj < key) {
+        ++rank;
+        before += s_cnt[j];
+      }
+    }
+    const int a = i / side - d.nn, b = i % side - d.nn;
+    c_beg[rank] = s_beg[i];
+    c_cum[rank] = before;
+    c_ab[rank] = ((a + 64) << 8) | (b + 64);
+  }
+  if (t == 0) {
+    int k = 0, tot = 0, n0 = 0, n1 = 0;
+    for (int j = 0; j < ncell; ++j) {
+      const int kj = s_key[j];
+      if (kj < 0) continue;
+      const int c = kj / ncell;
+      ++k;
+      tot += s_cnt[j];
+      n0 += c == 0 ? 1 : 0;
+      n1 += c <= 1 ? 1 : 0;
+    }
+    c_cum[k] = tot;
+    ncls[0] = n0;
+    ncls[1] = n1;
+    ncls[2] = k;
+  }
+  __syncthreads();
+}
+
 #ifndef LSIM_FAR_GROUP
 #define LSIM_FAR_GROUP 4
 #endif
@@ -652,7 +730,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   __shared__ int shi[NT / kWave];
   // this row's neighbourhood cells: far cells first, then the 3x3 near cells, in (a, b) order
   __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
-  __shared__ int n_far_cells, n_center_end, n_cells_tot;
+  __shared__ int ncls[3];  // far cells, + the centre cell, + the 8 side cells
 
   // XCD-aware mapping: workgroups are dispatched round-robin over the XCDs, so workgroup i runs
   // on XCD i % kXcd.  Workgroups 0..P-1 map one-to-one onto rows 0..P-1: XCD x receives the
@@ -674,43 +752,27 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   const int lane = t & (kWave - 1);
   const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
   const int rstride = nout + kRowPad;
-  for (int o = t; o < WAVES * rstride; o += NT) acc[o] = 0.f;
   const int cell = row_cell[r];
-  const int px = cell % d.gx - d.halo;
-  const int py = (cell / d.gx) % d.gy - d.halo;
   const int evp = cell / (d.gx * d.gy);  // ev * ntpc + plane
   const int plane = evp % d.ntpc;
   if (t == 0) {
+    const int px = cell % d.gx - d.halo;
+    const int py = (cell / d.gx) % d.gy - d.halo;
     out.pixel_ids[r] = ((int64_t)evp * d.ny + py) * d.nx + px;
-    int k = 0;
-    for (int pass = 0; pass < 3; ++pass) {  // far cells, the centre cell, the 8 side cells
-      for (int a = -d.nn; a <= d.nn; ++a) {
-        for (int b = -d.nn; b <= d.nn; ++b) {
-          const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
-          const int kind = !near ? 0 : ((a == 0 && b == 0) ? 1 : 2);
-          if (kind != pass) continue;
-          const int qx = px - a, qy = py - b;
-          if (qx < -d.halo || qx >= d.nx + d.halo || qy < -d.halo || qy >= d.ny + d.halo) continue;
-          const int64_t qc = ((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo);
-          const int2 rg = rng[qc];
-          if (rg.y <= rg.x) continue;
-          c_beg[k] = rg.x;
-          c_cum[k + 1] = rg.y - rg.x;
-          c_ab[k] = ((a + 64) << 8) | (b + 64);
-          ++k;
-        }
-      }
-      if (pass == 0) n_far_cells = k;
-      if (pass == 1) n_center_end = k;
-    }
-    n_cells_tot = k;
-    c_cum[0] = 0;
-    for (int i = 1; i <= k; ++i) c_cum[i] += c_cum[i - 1];
   }
+  // far cells first, then the centre cell, then the 8 side cells (the partial rows are scratch)
+  list_cells<NT>(
+      d, rng, cell,
+      [](int a, int b) {
+        const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
+        return !near ? 0 : ((a == 0 && b == 0) ? 1 : 2);
+      },
+      reinterpret_cast<int*>(acc), c_beg, c_cum, c_ab, ncls);
+  for (int o = t; o < WAVES * rstride; o += NT) acc[o] = 0.f;
   __syncthreads();
 
   float* accw = acc + wid * rstride;
-  const int nfc = n_far_cells, nct = n_cells_tot;
+  const int nfc = ncls[0], n_center_end = ncls[1], nct = ncls[2];
   const int tot = c_cum[nct];
   // far pairs: cell by cell, kFarGroup independent segments per step (wave w takes the cell's
   // segments w, w + WAVES, ...); then the near pairs (windows of at most 187 samples)
