@@ -245,69 +245,103 @@ __device__ __forceinline__ void axis_cells(int u, int a, int nsb, const float* w
   }
 }
 
-// Contribution of segment record R (chain factors A) to the 7 gradients through pixel row
-// (main pixel + (a, b)): the neighbour stripe unless (a, b) = (0, 0), and the diffusion bins
-// landing on the row when |a|, |b| <= 1.  Forward placement (sim_jax.py:170-261): (1-f) V at
-// o = S + j, f V at o = S - 1 + j, (1-f) Delta at S - 1 and f Delta at S - 2, S = nt - L - k0.
-template <int NB, bool NEAR>
-__device__ __forceinline__ void pair_adjoint(const Dev& d, const Tables& tb, const SegRec& R,
-                                             const AdjRec* __restrict__ Ap, int a, int b, const RowG& G, double* g) {
-  const int L = d.L, nt = d.nt, nout = d.nout, nsb = d.nsb;
-  const int S = nt - L - R.k0;
+// Window placement of one (segment, row) pair (sim_jax.py:170-261): (1-f) V at o = S + j, f V at
+// o = S - 1 + j, (1-f) Delta at S - 1 and f Delta at S - 2, S = nt - L - k0; G's values at the two
+// correction ticks and the tick ranges of the two windows kept in the row.
+struct PairCtx {
+  int S, lo1, hi1, lo0, hi0;
+  double f, ne, gc0, gc1, gc;
+};
+
+// false when G vanishes from tick S - 2 on (the pair has no gradient)
+__device__ __forceinline__ bool pair_ctx(const Dev& d, const SegRec& R, const RowG& G, PairCtx& c) {
+  const int L = d.L, nout = d.nout;
+  const int S = d.nt - L - R.k0;
   const int o2 = S - 2 > 0 ? S - 2 : 0;
-  if (G.qlut[o2] >= G.np) return;  // G vanishes from tick S - 2 on
-  const double f = (double)R.f, ne = (double)R.ne;
-  const double gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? G.at(S - 2) : 0.0;
-  const double gc0 = (S - 1 >= 0 && S - 1 <= nout - 2) ? G.at(S - 1) : 0.0;
-  const double gc = (1.0 - f) * gc0 + f * gc1;
-  const int lo1 = S > 1 ? S : 1, hi1 = (S + L - 1) < (nout - 1) ? (S + L - 1) : (nout - 1);
-  const int lo0 = (S - 1) > 1 ? (S - 1) : 1, hi0 = (S + L - 2) < (nout - 1) ? (S + L - 2) : (nout - 1);
+  if (G.qlut[o2] >= G.np) return false;
+  c.S = S;
+  c.f = (double)R.f;
+  c.ne = (double)R.ne;
+  c.gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? G.at(S - 2) : 0.0;
+  c.gc0 = (S - 1 >= 0 && S - 1 <= nout - 2) ? G.at(S - 1) : 0.0;
+  c.gc = (1.0 - c.f) * c.gc0 + c.f * c.gc1;
+  c.lo1 = S > 1 ? S : 1;
+  c.hi1 = (S + L - 1) < (nout - 1) ? (S + L - 1) : (nout - 1);
+  c.lo0 = (S - 1) > 1 ? (S - 1) : 1;
+  c.hi0 = (S + L - 2) < (nout - 1) ? (S + L - 2) : (nout - 1);
+  return true;
+}
+
+// The neighbour stripe of pixel offset (a, b) != (0, 0): d loss / d n_e (gN) and / d f (gF).
+__device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, const SegRec& R, const PairCtx& c, int a,
+                                               int b, const RowG& G, double& gN, double& gF) {
+  const int L = d.L, nsb = d.nsb;
   const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
-  double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
-  if (a != 0 || b != 0) {
-    const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
-    double y1, y0;
-    g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, S, lo1, hi1, lo0, hi0, &y1, &y0);
-    const float2 D = tb.DN[sidx * nt + R.k0];
+  const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
+  double y1, y0;
+  g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, c.S, c.lo1, c.hi1, c.lo0, c.hi0, &y1, &y0);
+  const float2 D = tb.DN[sidx * d.nt + R.k0];
+  const double corr = (double)D.x - c.f * (double)D.y;
+  gN += (1.0 - c.f) * y1 + c.f * y0 + c.gc * corr;
+  gF += c.ne * (y0 - y1 + (c.gc1 - c.gc0) * corr - c.gc * (double)D.y);
+}
+
+// The diffusion bins of x-bin i (and every later x-bin with the same LUT cell, merged into it)
+// landing on pixel offset (a, b), |a|, |b| <= 1: d loss / d (n_e, f, sigma_T, lambda) summed over
+// the y bins.  false (nothing added) when bin i misses column a or an earlier x-bin has its cell.
+template <int NB>
+__device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, const SegRec& R, const AdjRec* __restrict__ Ap,
+                                             const PairCtx& c, int a, int b, int i, const RowG& G, double& gN,
+                                             double& gF, double& gS, double& gL) {
+  constexpr int SYM = (NB - 1) / 2;
+  const int L = d.L, nsb = d.nsb, cm = d.cm;
+  const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
+  const int vi = ux + i - SYM;
+  const int di = vi < 0 ? -1 : (vi >= nsb ? 1 : 0);
+  if (di != a) return false;
+  const int cx = offset_bin(vi - di * nsb, 0, nsb);
+  float WXi = 0.f, DWXi = 0.f;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int v = ux + k - SYM;
+    const int dv = v < 0 ? -1 : (v >= nsb ? 1 : 0);
+    if (dv != a || offset_bin(v - dv * nsb, 0, nsb) != cx) continue;
+    if (k < i) return false;  // merged into x-bin k
+    WXi += R.wx[k];
+    DWXi += Ap->dwx[k];
+  }
+  bool oky[NB];
+  int cyv[NB];
+  float WY[NB], DWY[NB];
+  axis_cells<NB>(uy, b, nsb, R.wy, Ap->dwy, oky, cyv, WY, DWY);
+  const int64_t tstride = (int64_t)cm * cm * (L + 1);
+  const double la = (double)R.la, lb = (double)R.lb, lc = (double)R.lc;
+  const double da = Ap->dl[0], db = Ap->dl[1], dc = Ap->dl[2];
+  const double f = c.f, ne = c.ne;
+#pragma unroll
+  for (int jj = 0; jj < NB; ++jj) {
+    if (!oky[jj]) continue;
+    const int64_t cell = ((int64_t)R.idx * cm + cx) * cm + cyv[jj];
+    double y1[3], y0[3];
+    g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c.S, c.lo1, c.hi1, c.lo0, c.hi0, y1, y0);
+    const float2 D = tb.DM[cell * d.nt + R.k0];
     const double corr = (double)D.x - f * (double)D.y;
-    gN += (1.0 - f) * y1 + f * y0 + gc * corr;
-    gF += ne * (y0 - y1 + (gc1 - gc0) * corr - gc * (double)D.y);
+    const double wxy = (double)WXi * (double)WY[jj];
+    const double dwxy = (double)DWXi * (double)WY[jj] + (double)WXi * (double)DWY[jj];
+    const double Y1 = la * y1[0] + lb * y1[1] + lc * y1[2];
+    const double Y0 = la * y0[0] + lb * y0[1] + lc * y0[2];
+    const double gn = (1.0 - f) * Y1 + f * Y0 + c.gc * corr;  // d loss / d n_bin per unit weight
+    gN += gn * wxy;
+    gS += gn * ne * dwxy;
+    gF += ne * wxy * (Y0 - Y1 + (c.gc1 - c.gc0) * corr - c.gc * (double)D.y);
+    gL += ne * wxy * ((1.0 - f) * (da * y1[0] + db * y1[1] + dc * y1[2]) + f * (da * y0[0] + db * y0[1] + dc * y0[2]));
   }
-  constexpr bool near = NEAR;  // the caller passes the 3x3 near cells only to the NEAR instantiation
-  if (near) {
-    const int cm = d.cm;
-    const int64_t tstride = (int64_t)cm * cm * (L + 1);
-    const double la = (double)R.la, lb = (double)R.lb, lc = (double)R.lc;
-    const double da = Ap->dl[0], db = Ap->dl[1], dc = Ap->dl[2];
-    bool okx[NB], oky[NB];
-    int cxv[NB], cyv[NB];
-    float WX[NB], DWX[NB], WY[NB], DWY[NB];
-    axis_cells<NB>(ux, a, nsb, R.wx, Ap->dwx, okx, cxv, WX, DWX);
-    axis_cells<NB>(uy, b, nsb, R.wy, Ap->dwy, oky, cyv, WY, DWY);
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      if (!okx[i]) continue;
-#pragma unroll
-      for (int jj = 0; jj < NB; ++jj) {
-        if (!oky[jj]) continue;
-        const int64_t cell = ((int64_t)R.idx * cm + cxv[i]) * cm + cyv[jj];
-        double y1[3], y0[3];
-        g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, S, lo1, hi1, lo0, hi0, y1, y0);
-        const float2 D = tb.DM[cell * nt + R.k0];
-        const double corr = (double)D.x - f * (double)D.y;
-        const double wxy = (double)WX[i] * (double)WY[jj];
-        const double dwxy = (double)DWX[i] * (double)WY[jj] + (double)WX[i] * (double)DWY[jj];
-        const double Y1 = la * y1[0] + lb * y1[1] + lc * y1[2];
-        const double Y0 = la * y0[0] + lb * y0[1] + lc * y0[2];
-        const double gn = (1.0 - f) * Y1 + f * Y0 + gc * corr;  // d loss / d n_bin per unit weight
-        gN += gn * wxy;
-        gS += gn * ne * dwxy;
-        gF += ne * wxy * (Y0 - Y1 + (gc1 - gc0) * corr - gc * (double)D.y);
-        gL += ne * wxy * ((1.0 - f) * (da * y1[0] + db * y1[1] + dc * y1[2]) +
-                          f * (da * y0[0] + db * y0[1] + dc * y0[2]));
-      }
-    }
-  }
+  return true;
+}
+
+// Chain rule to the 7 parameters from the pair's d loss / d (n_e, f, sigma_T, lambda).
+__device__ __forceinline__ void add_chain(const AdjRec* __restrict__ Ap, double gN, double gF, double gS, double gL,
+                                          bool near, double* g) {
   g[0] += gN * Ap->fN[0];
   g[1] += gN * Ap->fN[1];
   g[2] += gN * Ap->fN[2] + gF * Ap->fF;
@@ -326,7 +360,7 @@ __device__ __forceinline__ void pair_adjoint(const Dev& d, const Tables& tb, con
 // neighbour-only cells (|a| or |b| > 1; little register state, high occupancy), NEAR = true the
 // 3x3 cells with the diffusion bins.  Rows without gradient (no valid hit) exit at once.
 template <int NB, int WAVES, bool NEAR>
-__global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, const SegRec* __restrict__ rec,
+__global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_rows_adj(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const AdjRec* __restrict__ adj, const int2* __restrict__ rng,
                                                             const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
@@ -383,14 +417,28 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj(Dev d, Tables tb, co
   const RowG G{spx, spc, sdc, qlut, np};
   const int nct = ncls[2];
   const int tot = c_cum[nct];
+  // work items: a far pair is one item (its stripe); a near pair is NB + 1 items on consecutive
+  // threads, x-bin rows 0..NB-1 of its diffusion bins and its stripe
+  constexpr int NI = NEAR ? NB + 1 : 1;
   int cur = 0;
-  for (int i = t; i < tot; i += NT) {
+  for (int it = t; it < tot * NI; it += NT) {
+    const int i = it / NI, part = it - i * NI;
     while (i >= c_cum[cur + 1]) ++cur;
     const int s = c_beg[cur] + (i - c_cum[cur]);
     const int ab = c_ab[cur];
+    const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
     const SegRec R = rec[s];
     if (R.ne == 0.f) continue;
-    pair_adjoint<NB, NEAR>(d, tb, R, adj + s, (ab >> 8) - 64, (ab & 255) - 64, G, g);
+    PairCtx pc;
+    if (!pair_ctx(d, R, G, pc)) continue;
+    double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
+    if (part == NB || !NEAR) {
+      if (a == 0 && b == 0) continue;
+      stripe_adjoint(d, tb, R, pc, a, b, G, gN, gF);
+    } else if (!xbin_adjoint<NB>(d, tb, R, adj + s, pc, a, b, part, G, gN, gF, gS, gL)) {
+      continue;
+    }
+    add_chain(adj + s, gN, gF, gS, gL, NEAR, g);
   }
   const int lane = t & (kWave - 1), wid = t / kWave;
 #pragma unroll
