@@ -288,10 +288,11 @@ __device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, c
 
 // The diffusion bins of x-bin i (and every later x-bin with the same LUT cell, merged into it)
 // landing on pixel offset (a, b), |a|, |b| <= 1: d loss / d (n_e, f, sigma_T, lambda) summed over
-// the y bins.  false (nothing added) when bin i misses column a or an earlier x-bin has its cell.
+// the y bins (only y-bin jsel when jsel >= 0).  false (nothing added) when bin i misses column a
+// or an earlier x-bin has its cell.
 template <int NB>
 __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, const SegRec& R, const AdjRec* __restrict__ Ap,
-                                             const PairCtx& c, int a, int b, int i, const RowG& G, double& gN,
+                                             const PairCtx& c, int a, int b, int i, int jsel, const RowG& G, double& gN,
                                              double& gF, double& gS, double& gL) {
   constexpr int SYM = (NB - 1) / 2;
   const int L = d.L, nsb = d.nsb, cm = d.cm;
@@ -320,7 +321,7 @@ __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, con
   const double f = c.f, ne = c.ne;
 #pragma unroll
   for (int jj = 0; jj < NB; ++jj) {
-    if (!oky[jj]) continue;
+    if (!oky[jj] || (jsel >= 0 && jj != jsel)) continue;
     const int64_t cell = ((int64_t)R.idx * cm + cx) * cm + cyv[jj];
     double y1[3], y0[3];
     g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c.S, c.lo1, c.hi1, c.lo0, c.hi0, y1, y0);
@@ -352,6 +353,11 @@ __device__ __forceinline__ void add_chain(const AdjRec* __restrict__ Ap, double 
     g[6] += gS * Ap->fS[1];
   }
 }
+
+#ifndef LSIM_ADJ_BIN_ITEMS
+#define LSIM_ADJ_BIN_ITEMS 0
+#endif
+constexpr bool kAdjBinItems = LSIM_ADJ_BIN_ITEMS != 0;  // one work item per diffusion bin (else per x-bin row)
 
 // Stage 2, row-owned: one workgroup per pixel row holds the row's step function in LDS and walks
 // the segments of its neighbourhood cells (the same pairs the forward's waveform kernel visits),
@@ -417,9 +423,10 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
   const RowG G{spx, spc, sdc, qlut, np};
   const int nct = ncls[2];
   const int tot = c_cum[nct];
-  // work items: a far pair is one item (its stripe); a near pair is NB + 1 items on consecutive
-  // threads, x-bin rows 0..NB-1 of its diffusion bins and its stripe
-  constexpr int NI = NEAR ? NB + 1 : 1;
+  // work items: a far pair is one item (its stripe); a near pair is NX + 1 items on consecutive
+  // threads, its diffusion bins by x-bin row (NX = NB) or one by one (NX = NB * NB), and its stripe
+  constexpr int NX = kAdjBinItems ? NB * NB : NB;
+  constexpr int NI = NEAR ? NX + 1 : 1;
   int cur = 0;
   for (int it = t; it < tot * NI; it += NT) {
     const int i = it / NI, part = it - i * NI;
@@ -432,10 +439,11 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
     PairCtx pc;
     if (!pair_ctx(d, R, G, pc)) continue;
     double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
-    if (part == NB || !NEAR) {
+    if (part == NX || !NEAR) {
       if (a == 0 && b == 0) continue;
       stripe_adjoint(d, tb, R, pc, a, b, G, gN, gF);
-    } else if (!xbin_adjoint<NB>(d, tb, R, adj + s, pc, a, b, part, G, gN, gF, gS, gL)) {
+    } else if (!xbin_adjoint<NB>(d, tb, R, adj + s, pc, a, b, kAdjBinItems ? part / NB : part,
+                                 kAdjBinItems ? part % NB : -1, G, gN, gF, gS, gL)) {
       continue;
     }
     add_chain(adj + s, gN, gF, gS, gL, NEAR, g);
