@@ -392,25 +392,21 @@ __device__ __forceinline__ void store_window(float* accw, int lane, float v0, fl
 }
 
 // Neighbour stripe only (|a| or |b| > 1): raw response window, undiffused charge.  L <= 187.
-// Split into `far_load` (issue the window and correction loads) and `far_store` so a wave keeps
-// several independent pairs in flight.
+// The loads (`far_load`) are issued for a group of pairs before any of their stores, so a wave
+// keeps several independent pairs in flight.
 struct FarLoads {
   float t0, t1, t2;
   float D;  // boundary correction Delta of the pair (charge included)
 };
 
-// Window loads of a neighbour-only pair: tnp = padded raw windows + this lane's byte offset + the
-// cell constant; off = the segment's part of the row offset (bytes).
+// Window loads of a neighbour-only pair: tnp = padded raw windows + this lane's byte offset;
+// off = the pair's row offset (bytes).
 __device__ __forceinline__ void far_load(const char* __restrict__ tnp, int off, float delta, FarLoads& q) {
   const float* t = reinterpret_cast<const float*>(tnp + off);
   q.D = delta;
   q.t0 = t[0];
   q.t1 = t[kChunk];
   q.t2 = t[2 * kChunk];
-}
-
-__device__ __forceinline__ void far_store(const Dev& d, const PairRec& P, const FarLoads& q, float* accw, int lane) {
-  store_window(accw, lane, q.t0, q.t1, q.t2, P.ne, 1.f - P.f, P.f, q.D, P.S, d.L, d.nout, (P.flags & 1) != 0);
 }
 
 // Distinct LUT cells of a segment's diffusion bins that land on pixel offset a along one axis, with
@@ -774,47 +770,65 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   float* accw = acc + wid * rstride;
   const int nfc = ncls[0], n_center_end = ncls[1], nct = ncls[2];
   const int tot = c_cum[nct];
-  // far pairs: cell by cell, kFarGroup independent segments per step (wave w takes the cell's
-  // segments w, w + WAVES, ...); then the near pairs (windows of at most 187 samples)
-  for (int c = 0; c < nfc; ++c) {
-    const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
-    const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
-    const int ab = __builtin_amdgcn_readfirstlane(c_ab[c]);
-    const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
-    const float* dt = dtot + (int64_t)((a + d.nn) * (2 * d.nn + 1) + (b + d.nn)) * nseg + beg;
-    const PairRec* pr = prs + beg;
-    // row offset of the pair = cell constant + ma U + za U0 + mb V + zb V0 (bytes)
-    const int W4 = kWinPad * 4, nsb = d.nsb;
-    int kc = 0, ma = 0, za = 0, mb = 0, zb = 0;
-    if (a > 0) { kc += (nsb * a + nsb / 2 - 1) * d.ly * W4; ma = -1; }
-    else if (a < 0) { kc += (nsb * -a - nsb / 2) * d.ly * W4; ma = 1; }
-    else za = 1;
-    if (b > 0) { kc += (nsb * b + nsb / 2 - 1) * W4; mb = -1; }
-    else if (b < 0) { kc += (nsb * -b - nsb / 2) * W4; mb = 1; }
-    else zb = 1;
-    const char* tnp = reinterpret_cast<const char*>(tb.TNP + lane) + kc;
-    for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
-      PairRec P[kFarGroup];
-      float D[kFarGroup];
-#pragma unroll
-      for (int g = 0; g < kFarGroup; ++g) {
-        const int j = k + g * WAVES;
-        const int jj = j < cnt ? j : k;
-        P[g] = pr[jj];
-        D[g] = dt[jj];
-        if (j >= cnt) {  // padding slot of the last group: adds exactly zero
-          P[g].ne = 0.f;
-          D[g] = 0.f;
+  // far pairs, flattened over the far cells in blocks of 64: wave w takes blocks w, w + WAVES, ...
+  // Lane l loads pair 64 b + l's record, boundary correction and row offset into registers (one
+  // memory round trip per block instead of one per pair group), then the wave walks the block
+  // kFarGroup pairs at a time, reading each pair's values from its lane.
+  {
+    const int nfar = c_cum[nfc];
+    const int W4 = kWinPad * 4, nsb = d.nsb, side = 2 * d.nn + 1;
+    const char* tnp = reinterpret_cast<const char*>(tb.TNP + lane);
+    for (int b0 = wid * kWave; b0 < nfar; b0 += WAVES * kWave) {
+      const int p = b0 + lane;
+      // padding lanes: an interior window at S = 2 with zero charge and correction (adds +0)
+      int rS = (2 << 1) | 1, roff = 0;
+      float rne = 0.f, rf = 0.f, rD = 0.f;
+      if (p < nfar) {
+        int lo = 0, hi = nfc - 1;  // the far cell holding p: last c with c_cum[c] <= p
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (c_cum[mid] <= p) lo = mid;
+          else hi = mid - 1;
         }
+        const int sg = c_beg[lo] + (p - c_cum[lo]);
+        const int ab = c_ab[lo];
+        const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+        // row offset = cell constant + ma U + za U0 + mb V + zb V0 (bytes)
+        int kc = 0, ma = 0, za = 0, mb = 0, zb = 0;
+        if (a > 0) { kc += (nsb * a + nsb / 2 - 1) * d.ly * W4; ma = -1; }
+        else if (a < 0) { kc += (nsb * -a - nsb / 2) * d.ly * W4; ma = 1; }
+        else za = 1;
+        if (b > 0) { kc += (nsb * b + nsb / 2 - 1) * W4; mb = -1; }
+        else if (b < 0) { kc += (nsb * -b - nsb / 2) * W4; mb = 1; }
+        else zb = 1;
+        const PairRec P = prs[sg];
+        rD = dtot[(int64_t)((a + d.nn) * side + (b + d.nn)) * nseg + sg];
+        roff = kc + ma * P.U + za * P.U0 + mb * P.V + zb * P.V0;
+        rS = (P.S << 1) | (P.flags & 1);
+        rne = P.ne;
+        rf = P.f;
       }
-      FarLoads q[kFarGroup];
+      const int n = __builtin_amdgcn_readfirstlane(nfar - b0 < kWave ? nfar - b0 : kWave);
+      for (int i = 0; i < n; i += kFarGroup) {
+        FarLoads q[kFarGroup];
+        int S[kFarGroup];
+        float ne[kFarGroup], f[kFarGroup];
 #pragma unroll
-      for (int g = 0; g < kFarGroup; ++g)
-        far_load(tnp, ma * P[g].U + za * P[g].U0 + mb * P[g].V + zb * P[g].V0, D[g], q[g]);
-      // unconditional stores: a branch on the charge would let the compiler sink a pair's loads
-      // into it and wait on them there instead of overlapping them with the other pairs'
+        for (int g = 0; g < kFarGroup; ++g) {
+          const int j = i + g;  // < 64: i < n <= 64 and i is a multiple of kFarGroup
+          S[g] = __builtin_amdgcn_readlane(rS, j);
+          ne[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rne), j));
+          f[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rf), j));
+          const float D = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rD), j));
+          far_load(tnp, __builtin_amdgcn_readlane(roff, j), D, q[g]);
+        }
+        // unconditional stores: a branch on the charge would let the compiler sink a pair's loads
+        // into it and wait on them there instead of overlapping them with the other pairs'
 #pragma unroll
-      for (int g = 0; g < kFarGroup; ++g) far_store(d, P[g], q[g], accw, lane);
+        for (int g = 0; g < kFarGroup; ++g)
+          store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, ne[g], 1.f - f[g], f[g], q[g].D, S[g] >> 1, d.L, d.nout,
+                       (S[g] & 1) != 0);
+      }
     }
   }
   // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
