@@ -446,19 +446,26 @@ __device__ __forceinline__ void axis_merge(int u, int a, int nsb, const float* w
   }
 }
 
-// |a|, |b| <= 1: the diffusion bins landing on this row (interpolated templates) plus the
-// neighbour stripe unless (a, b) = (0, 0).  TM3[idx][cx][cy][j] holds (T_{idx-1}, T_idx,
-// T_{idx+1}) so one 12-byte load serves the three Lagrange nodes.  Bin i of the segment lies
-// delta_i = floor((ux + i - SYM) / nsb) in {-1, 0, 1} pixels from the main pixel.
-template <int NB, bool CENTER>
-__device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, float delta,
-                                          float* accw, int lane) {
+// A segment's diffusion bins along one axis reach pixel offset a (|a| <= 1) when some bin
+// position u + i - SYM, i < NB, lies in pixel a: always for a = 0, else only near the edge.
+template <int NB>
+__device__ __forceinline__ bool bins_reach(int u, int a, int nsb) {
   constexpr int SYM = (NB - 1) / 2;
-  const int nsb = d.nsb, L = d.L, cm = d.cm;
+  return a == 0 ? true : (a > 0 ? u + (NB - 1 - SYM) >= nsb : u - SYM < 0);
+}
+
+// Window of the diffusion bins (interpolated templates) of segment R that land on a side row,
+// pixel offset (a, b) != (0, 0) with |a|, |b| <= 1, charge included.  TM3[idx][cx][cy][j] holds
+// (T_{idx-1}, T_idx, T_{idx+1}) so one 12-byte load serves the three Lagrange nodes.  Bin i of the
+// segment lies floor((u + i - SYM) / nsb) in {-1, 0, 1} pixels from the main pixel.
+template <int NB>
+__device__ __forceinline__ void side_bins(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, int lane,
+                                          float& v0, float& v1, float& v2) {
+  constexpr int SYM = (NB - 1) / 2;
+  const int nsb = d.nsb, cm = d.cm;
   const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
   int cxs[NB], cys[NB];
   float wxs[NB], wys[NB];
-  bool anyx = false, anyy = false;
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     int vx = ux + i - SYM, vy = uy + i - SYM;
@@ -470,50 +477,29 @@ __device__ __forceinline__ void near_pair(const Dev& d, const Tables& tb, const 
     cys[i] = offset_bin(vy, 0, nsb);
     wxs[i] = (dx == a) ? R.ne * R.wx[i] : 0.f;  // zero weight: the bin lands on another pixel
     wys[i] = (dy == b) ? R.wy[i] : 0.f;
-    anyx |= (dx == a);
-    anyy |= (dy == b);
   }
-  const bool has_bins = anyx && anyy;
-  constexpr bool has_nb = !CENTER;
-  if (!has_nb && !has_bins) return;
-  const int idx = R.idx, k0 = R.k0;
-  const float f = R.f, omf = 1.f - f;
   const float la = R.la, lb = R.lb, lc = R.lc;
-  float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-  if (has_nb) {
-    const int cxN = offset_bin(ux, a, nsb);
-    const int cyN = offset_bin(uy, b, nsb);
-    const int64_t sidx = (int64_t)cxN * d.ly + cyN;
-    const float* TNp = tb.TNP + sidx * kWinPad + lane;
-    const float ne = R.ne;
-    v0 = ne * TNp[0];
-    v1 = ne * TNp[kChunk];
-    v2 = ne * TNp[2 * kChunk];
-  }
-  if (has_bins) {
-    const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * kWinPad + lane;
+  v0 = v1 = v2 = 0.f;
+  const float3* T3 = tb.TM3 + (int64_t)R.idx * cm * cm * kWinPad + lane;
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      // one uniform branch per row of bins: the row's 3 * NB loads overlap, rows do not (bounds
-      // the live registers); bins of the row that land elsewhere load but add zero
-      if (wxs[i] != 0.f) {
+  for (int i = 0; i < NB; ++i) {
+    // one uniform branch per row of bins: the row's 3 * NB loads overlap, rows do not (bounds
+    // the live registers); bins of the row that land elsewhere load but add zero
+    if (wxs[i] != 0.f) {
 #pragma unroll
-        for (int jj = 0; jj < NB; ++jj) {
-          const float c = wxs[i] * wys[jj];
-          const int64_t coff = (int64_t)cxs[i] * cm + cys[jj];
-          const float3* tp = T3 + coff * kWinPad;
-          const float3 q0 = tp[0];
-          const float3 q1 = tp[kChunk];
-          const float3 q2 = tp[2 * kChunk];
-          v0 += c * (q0.y * lb + q0.x * la + q0.z * lc);
-          v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
-          v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
-        }
+      for (int jj = 0; jj < NB; ++jj) {
+        const float c = wxs[i] * wys[jj];
+        const int64_t coff = (int64_t)cxs[i] * cm + cys[jj];
+        const float3* tp = T3 + coff * kWinPad;
+        const float3 q0 = tp[0];
+        const float3 q1 = tp[kChunk];
+        const float3 q2 = tp[2 * kChunk];
+        v0 += c * (q0.y * lb + q0.x * la + q0.z * lc);
+        v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
+        v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
       }
     }
   }
-  const int S = d.nt - L - k0;
-  store_window(accw, lane, v0, v1, v2, 1.f, omf, f, delta, S, L, d.nout, S >= 2 && S + L <= d.nout);
 }
 
 // Segment-parallel window of the diffusion bins that land on the segment's own (main) pixel:
@@ -756,35 +742,37 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     const int py = (cell / d.gx) % d.gy - d.halo;
     out.pixel_ids[r] = ((int64_t)evp * d.ny + py) * d.nx + px;
   }
-  // far cells first, then the centre cell, then the 8 side cells (the partial rows are scratch)
+  // far cells first, then the 8 side cells, then the centre cell (the partial rows are scratch)
   list_cells<NT>(
       d, rng, cell,
       [](int a, int b) {
         const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
-        return !near ? 0 : ((a == 0 && b == 0) ? 1 : 2);
+        return !near ? 0 : ((a == 0 && b == 0) ? 2 : 1);
       },
       reinterpret_cast<int*>(acc), c_beg, c_cum, c_ab, ncls);
   for (int o = t; o < WAVES * rstride; o += NT) acc[o] = 0.f;
   __syncthreads();
 
   float* accw = acc + wid * rstride;
-  const int nfc = ncls[0], n_center_end = ncls[1], nct = ncls[2];
-  const int tot = c_cum[nct];
-  // far pairs, flattened over the far cells in blocks of 64: wave w takes blocks w, w + WAVES, ...
-  // Lane l loads pair 64 b + l's record, boundary correction and row offset into registers (one
-  // memory round trip per block instead of one per pair group), then the wave walks the block
-  // kFarGroup pairs at a time, reading each pair's values from its lane.
+  const int nsc = ncls[1], nct = ncls[2];  // far + side cells, all cells
+  // Neighbour stripes of the far and side pairs, flattened over those cells in blocks of 64: wave
+  // w takes blocks w, w + WAVES, ...  Lane l loads pair 64 b + l's record, boundary correction and
+  // row offset into registers (one memory round trip per block instead of one per pair group),
+  // then the wave walks the block kFarGroup pairs at a time, reading each pair's values from its
+  // lane.  A side pair's correction covers its diffusion bins too; the bins that land on a side
+  // row (segments near the pixel edge only) follow per block, picked out by a ballot.
   {
-    const int nfar = c_cum[nfc];
+    const int nfar = c_cum[nsc];
     const int W4 = kWinPad * 4, nsb = d.nsb, side = 2 * d.nn + 1;
     const char* tnp = reinterpret_cast<const char*>(tb.TNP + lane);
     for (int b0 = wid * kWave; b0 < nfar; b0 += WAVES * kWave) {
       const int p = b0 + lane;
       // padding lanes: an interior window at S = 2 with zero charge and correction (adds +0)
-      int rS = (2 << 1) | 1, roff = 0;
+      int rS = (2 << 1) | 1, roff = 0, rsg = 0, rab = 0;
       float rne = 0.f, rf = 0.f, rD = 0.f;
+      bool bins = false;
       if (p < nfar) {
-        int lo = 0, hi = nfc - 1;  // the far cell holding p: last c with c_cum[c] <= p
+        int lo = 0, hi = nsc - 1;  // the cell holding p: last c with c_cum[c] <= p
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
           if (c_cum[mid] <= p) lo = mid;
@@ -807,6 +795,10 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         rS = (P.S << 1) | (P.flags & 1);
         rne = P.ne;
         rf = P.f;
+        rsg = sg;
+        rab = ab;
+        bins = a >= -1 && a <= 1 && b >= -1 && b <= 1 && P.ne != 0.f && bins_reach<NB>((P.flags >> 8) & 255, a, nsb) &&
+               bins_reach<NB>((P.flags >> 16) & 255, b, nsb);
       }
       const int n = __builtin_amdgcn_readfirstlane(nfar - b0 < kWave ? nfar - b0 : kWave);
       for (int i = 0; i < n; i += kFarGroup) {
@@ -829,11 +821,20 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
           store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, ne[g], 1.f - f[g], f[g], q[g].D, S[g] >> 1, d.L, d.nout,
                        (S[g] & 1) != 0);
       }
+      for (uint64_t m = __ballot(bins); m != 0; m &= m - 1) {
+        const int j = __builtin_ctzll(m);
+        const int ab = __builtin_amdgcn_readlane(rab, j);
+        const SegRec R = rec[__builtin_amdgcn_readlane(rsg, j)];
+        float v0, v1, v2;
+        side_bins<NB>(d, tb, R, (ab >> 8) - 64, (ab & 255) - 64, lane, v0, v1, v2);
+        const int S = d.nt - d.L - R.k0;
+        store_window(accw, lane, v0, v1, v2, 1.f, 1.f - R.f, R.f, 0.f, S, d.L, d.nout, S >= 2 && S + d.L <= d.nout);
+      }
     }
   }
   // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
   const float* dtc = dtot + (int64_t)(d.nn * (2 * d.nn + 1) + d.nn) * nseg;
-  for (int c = nfc; c < n_center_end; ++c) {
+  for (int c = nsc; c < nct; ++c) {
     const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
     const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
     for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
@@ -855,20 +856,6 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
       for (int g = 0; g < kFarGroup; ++g)
         store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, sc[g], 1.f - P[g].f, P[g].f, sc[g] * q[g].D, P[g].S, d.L,
                      d.nout, (P[g].flags & 1) != 0);
-    }
-  }
-  {
-    const int side_begin = c_cum[n_center_end];
-    int cur = n_center_end;
-    for (int i = side_begin + wid; i < tot; i += WAVES) {
-      while (i >= c_cum[cur + 1]) ++cur;
-      const int s = __builtin_amdgcn_readfirstlane(c_beg[cur] + (i - c_cum[cur]));
-      const int ab = __builtin_amdgcn_readfirstlane(c_ab[cur]);
-      const SegRec R = rec[s];
-      if (R.ne == 0.f) continue;
-      const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
-      const float delta = dtot[(int64_t)((a + d.nn) * (2 * d.nn + 1) + (b + d.nn)) * nseg + s];
-      near_pair<NB, false>(d, tb, R, a, b, delta, accw, lane);
     }
   }
   __syncthreads();
