@@ -253,15 +253,15 @@ struct PairCtx {
   double f, ne, gc0, gc1, gc;
 };
 
-// false when G vanishes from tick S - 2 on (the pair has no gradient)
-__device__ __forceinline__ bool pair_ctx(const Dev& d, const SegRec& R, const RowG& G, PairCtx& c) {
+// G vanishes from tick S - 2 on: the pair has no gradient
+__device__ __forceinline__ bool pair_dead(int S, const RowG& G) { return G.qlut[S - 2 > 0 ? S - 2 : 0] >= G.np; }
+
+// (window start S, charge, fractional tick) -> the pair's placement; G must not vanish at S - 2
+__device__ __forceinline__ void pair_ctx(const Dev& d, int S, float ne, float f, const RowG& G, PairCtx& c) {
   const int L = d.L, nout = d.nout;
-  const int S = d.nt - L - R.k0;
-  const int o2 = S - 2 > 0 ? S - 2 : 0;
-  if (G.qlut[o2] >= G.np) return false;
   c.S = S;
-  c.f = (double)R.f;
-  c.ne = (double)R.ne;
+  c.f = (double)f;
+  c.ne = (double)ne;
   c.gc1 = (S - 2 >= 0 && S - 2 <= nout - 2) ? G.at(S - 2) : 0.0;
   c.gc0 = (S - 1 >= 0 && S - 1 <= nout - 2) ? G.at(S - 1) : 0.0;
   c.gc = (1.0 - c.f) * c.gc0 + c.f * c.gc1;
@@ -269,18 +269,17 @@ __device__ __forceinline__ bool pair_ctx(const Dev& d, const SegRec& R, const Ro
   c.hi1 = (S + L - 1) < (nout - 1) ? (S + L - 1) : (nout - 1);
   c.lo0 = (S - 1) > 1 ? (S - 1) : 1;
   c.hi0 = (S + L - 2) < (nout - 1) ? (S + L - 2) : (nout - 1);
-  return true;
 }
 
-// The neighbour stripe of pixel offset (a, b) != (0, 0): d loss / d n_e (gN) and / d f (gF).
-__device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, const SegRec& R, const PairCtx& c, int a,
+// The neighbour stripe of pixel offset (a, b) != (0, 0) of a segment at sub-bins (ux, uy):
+// d loss / d n_e (gN) and / d f (gF).
+__device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, int ux, int uy, const PairCtx& c, int a,
                                                int b, const RowG& G, double& gN, double& gF) {
   const int L = d.L, nsb = d.nsb;
-  const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
   const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
   double y1, y0;
   g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, c.S, c.lo1, c.hi1, c.lo0, c.hi0, &y1, &y0);
-  const float2 D = tb.DN[sidx * d.nt + R.k0];
+  const float2 D = tb.DN[sidx * d.nt + (d.nt - L - c.S)];  // k0 = nt - L - S
   const double corr = (double)D.x - c.f * (double)D.y;
   gN += (1.0 - c.f) * y1 + c.f * y0 + c.gc * corr;
   gF += c.ne * (y0 - y1 + (c.gc1 - c.gc0) * corr - c.gc * (double)D.y);
@@ -367,6 +366,7 @@ constexpr bool kAdjBinItems = LSIM_ADJ_BIN_ITEMS != 0;  // one work item per dif
 // 3x3 cells with the diffusion bins.  Rows without gradient (no valid hit) exit at once.
 template <int NB, int WAVES, bool NEAR>
 __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_rows_adj(Dev d, Tables tb, const SegRec* __restrict__ rec,
+                                                            const PairRec* __restrict__ prs,
                                                             const AdjRec* __restrict__ adj, const int2* __restrict__ rng,
                                                             const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
@@ -434,14 +434,29 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
     const int s = c_beg[cur] + (i - c_cum[cur]);
     const int ab = c_ab[cur];
     const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+    double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
+    PairCtx pc;
+    if (!NEAR) {
+      // the 32-byte pair record: window start first (most far pairs end where G has vanished)
+      const PairRec* P = prs + s;
+      const int S = P->S;
+      if (pair_dead(S, G)) continue;
+      const float ne = P->ne;
+      if (ne == 0.f) continue;
+      pair_ctx(d, S, ne, P->f, G, pc);
+      const int fl = P->flags;
+      stripe_adjoint(d, tb, (fl >> 8) & 255, (fl >> 16) & 255, pc, a, b, G, gN, gF);
+      add_chain(adj + s, gN, gF, gS, gL, false, g);
+      continue;
+    }
     const SegRec R = rec[s];
     if (R.ne == 0.f) continue;
-    PairCtx pc;
-    if (!pair_ctx(d, R, G, pc)) continue;
-    double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
-    if (part == NX || !NEAR) {
+    const int S = d.nt - d.L - R.k0;
+    if (pair_dead(S, G)) continue;
+    pair_ctx(d, S, R.ne, R.f, G, pc);
+    if (part == NX) {
       if (a == 0 && b == 0) continue;
-      stripe_adjoint(d, tb, R, pc, a, b, G, gN, gF);
+      stripe_adjoint(d, tb, R.uxy & 0xffff, R.uxy >> 16, pc, a, b, G, gN, gF);
     } else if (!xbin_adjoint<NB>(d, tb, R, adj + s, pc, a, b, kAdjBinItems ? part / NB : part,
                                  kAdjBinItems ? part % NB : -1, G, gN, gF, gS, gL)) {
       continue;

@@ -661,10 +661,10 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
     const dim3 grid((unsigned)max_rows), block(kAdjWaves * kWave);
     double* pnear = partial + max_rows * kNGrad;
 #define LSIM_ADJ_LAUNCH(NBV)                                                                                        \
-  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell,  \
-                     w.n_rows, max_rows, pieces, partial);                                                          \
-  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true>), grid, block, 0, st, d, tb, w.rec, adj, w.rng, w.row_cell,   \
-                     w.n_rows, max_rows, pieces, pnear)
+  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,       \
+                     w.row_cell, w.n_rows, max_rows, pieces, partial);                                              \
+  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,        \
+                     w.row_cell, w.n_rows, max_rows, pieces, pnear)
     if (d.nb == 5) {
       LSIM_ADJ_LAUNCH(5);
     } else if (d.nb == 3) {
