@@ -169,14 +169,48 @@ struct RowG {
   }
 };
 
+// Window placement of one (segment, row) pair (sim_jax.py:170-261): (1-f) V at o = S + j, f V at
+// o = S - 1 + j, (1-f) Delta at S - 1 and f Delta at S - 2, S = nt - L - k0; G's values at the two
+// correction ticks and the tick ranges of the two windows kept in the row.
+struct PairCtx {
+  int S, lo1, hi1, lo0, hi0;
+  bool interior;  // both windows whole: lo1 = S, hi1 = S + L - 1, lo0 = S - 1, hi0 = S + L - 2
+  double f, ne, gc0, gc1, gc;
+};
+
 // Window dot products of one (segment, row) pair, telescoped over G's breakpoints.  With
 // CX[k] = sum_{j<k} X[j] and F(o) = CX[o - S + sh + 1],
 //   sum_{o=lo}^{hi} G(o) X[o - S + sh] = G(hi) F(hi) - G(lo) F(lo - 1) + sum_{lo <= x_q < hi} dc_q F(x_q),
 // so each breakpoint inside the window costs one prefix-sum load per table.  y1: the (1-f) window
 // (sh = 0, o in [lo1, hi1]); y0: the f window (sh = 1, o in [lo0, hi0]).  NT tables at stride ts.
+// Whole windows take fewer loads: F(lo - 1) = CX[0] = 0, both F(hi) are CX[L], and a breakpoint
+// needs the adjacent CX[x_q - S + 1] (y1) and CX[x_q - S + 2] (y0).
 template <int NT>
-__device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__ CX, int64_t ts, int S, int lo1, int hi1,
-                                       int lo0, int hi0, double* y1, double* y0) {
+__device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__ CX, int64_t ts, const PairCtx& c, int L,
+                                       double* y1, double* y0) {
+  const int S = c.S, lo1 = c.lo1, hi1 = c.hi1, lo0 = c.lo0, hi0 = c.hi0;
+  if (c.interior) {
+    const double gh1 = G.at(hi1), gh0 = G.at(hi0);
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const double cl = CX[k * ts + L];
+      y1[k] = gh1 * cl;
+      y0[k] = gh0 * cl;
+    }
+    for (int q = G.qlut[lo0]; q < G.np; ++q) {
+      const int xq = G.x[q];
+      if (xq >= hi1) break;
+      const double dq = (double)G.dc[q];
+      const int i = xq - S + 1;  // 0 <= i <= L - 1
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        const double v1 = CX[k * ts + i], v0 = CX[k * ts + i + 1];
+        if (xq >= lo1) y1[k] += dq * v1;
+        if (xq < hi0) y0[k] += dq * v0;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < NT; ++k) y1[k] = y0[k] = 0.0;
   const bool w1 = lo1 <= hi1, w0 = lo0 <= hi0;
@@ -245,14 +279,6 @@ __device__ __forceinline__ void axis_cells(int u, int a, int nsb, const float* w
   }
 }
 
-// Window placement of one (segment, row) pair (sim_jax.py:170-261): (1-f) V at o = S + j, f V at
-// o = S - 1 + j, (1-f) Delta at S - 1 and f Delta at S - 2, S = nt - L - k0; G's values at the two
-// correction ticks and the tick ranges of the two windows kept in the row.
-struct PairCtx {
-  int S, lo1, hi1, lo0, hi0;
-  double f, ne, gc0, gc1, gc;
-};
-
 // G vanishes from tick S - 2 on: the pair has no gradient
 __device__ __forceinline__ bool pair_dead(int S, const RowG& G) { return G.qlut[S - 2 > 0 ? S - 2 : 0] >= G.np; }
 
@@ -269,6 +295,7 @@ __device__ __forceinline__ void pair_ctx(const Dev& d, int S, float ne, float f,
   c.hi1 = (S + L - 1) < (nout - 1) ? (S + L - 1) : (nout - 1);
   c.lo0 = (S - 1) > 1 ? (S - 1) : 1;
   c.hi0 = (S + L - 2) < (nout - 1) ? (S + L - 2) : (nout - 1);
+  c.interior = c.lo1 == S && c.hi1 == S + L - 1 && c.lo0 == S - 1 && c.hi0 == S + L - 2;
 }
 
 // The neighbour stripe of pixel offset (a, b) != (0, 0) of a segment at sub-bins (ux, uy):
@@ -278,7 +305,7 @@ __device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, i
   const int L = d.L, nsb = d.nsb;
   const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
   double y1, y0;
-  g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, c.S, c.lo1, c.hi1, c.lo0, c.hi0, &y1, &y0);
+  g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, c, L, &y1, &y0);
   const float2 D = tb.DN[sidx * d.nt + (d.nt - L - c.S)];  // k0 = nt - L - S
   const double corr = (double)D.x - c.f * (double)D.y;
   gN += (1.0 - c.f) * y1 + c.f * y0 + c.gc * corr;
@@ -323,7 +350,7 @@ __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, con
     if (!oky[jj] || (jsel >= 0 && jj != jsel)) continue;
     const int64_t cell = ((int64_t)R.idx * cm + cx) * cm + cyv[jj];
     double y1[3], y0[3];
-    g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c.S, c.lo1, c.hi1, c.lo0, c.hi0, y1, y0);
+    g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c, L, y1, y0);
     const float2 D = tb.DM[cell * d.nt + R.k0];
     const double corr = (double)D.x - f * (double)D.y;
     const double wxy = (double)WXi * (double)WY[jj];
@@ -403,24 +430,35 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
     sdc[t] = pr->c[t] - (t + 1 < np ? pr->c[t + 1] : 0.f);
   }
   __syncthreads();
-  // qlut[o] = number of breakpoints below o = first q with x[q] >= o
-  for (int o = t; o < d.nout; o += NT) {
+  // qlut[o] = number of breakpoints below o = first q with x[q] >= o (np from nout on: G = 0 there)
+  for (int o = t; o < kMaxOut; o += NT) {
     int q = 0;
     for (int k = 0; k < np; ++k) q += spx[k] < o ? 1 : 0;
     qlut[o] = (uint8_t)q;
   }
-  // this pass's cells in (a, b) order
+  __syncthreads();  // qlut complete
+  // this pass's cells in (a, b) order, each with its live pairs only: window starts ascend within a
+  // cell and a pair is dead from the first start at which G has vanished, so the live pairs lead
+  const RowG G{spx, spc, sdc, qlut, np};
   list_cells<NT>(
       d, rng, row_cell[r],
       [](int a, int b) {
         const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
         return near == NEAR ? 0 : -1;
       },
+      [&](int b, int e) {
+        int lo = b, hi = e;  // the first dead pair of [b, e)
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (pair_dead(prs[mid].S, G)) hi = mid;
+          else lo = mid + 1;
+        }
+        return lo - b;
+      },
       cell_scratch, c_beg, c_cum, c_ab, ncls);
   double g[kNGrad];
 #pragma unroll
   for (int k = 0; k < kNGrad; ++k) g[k] = 0.0;
-  const RowG G{spx, spc, sdc, qlut, np};
   const int nct = ncls[2];
   const int tot = c_cum[nct];
   // work items: a far pair is one item (its stripe); a near pair is NX + 1 items on consecutive

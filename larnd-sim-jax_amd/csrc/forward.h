@@ -47,20 +47,42 @@ inline void fill(T* p, int64_t n, T v, hipStream_t st) {
 // 1. cell key per segment (input order)
 // ------------------------------------------------------------------------------------------
 
-// Key = main-pixel cell of the segment in the halo-extended grid, or n_cells (sorts last) for
-// segments that neither create a pixel row nor carry charge: padding rows (eventID < 0),
-// events out of range, and zero-charge segments whose main pixel is off the pixel plane.
-__global__ void k_keys(Dev d, SegIn in, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int* __restrict__ status) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= in.n) return;
+// Sort keys of a segment.  Cell key = main-pixel cell of the segment in the halo-extended grid,
+// or n_cells (sorts last) for segments that neither create a pixel row nor carry charge: padding
+// rows (eventID < 0), events out of range, and zero-charge segments whose main pixel is off the
+// pixel plane.  Window key = the first output tick S = nt - L - k0 of the segment's windows,
+// clamped to [0, nout + 2]: segments are ordered by cell, then by S (an adjoint row's live pairs,
+// those whose windows start before its gradient vanishes, then lead each cell).
+__device__ __forceinline__ void seg_keys(const Dev& d, const SegIn& in, int64_t i, uint32_t& cell_key, uint32_t& s_key,
+                                         int* status) {
   Phys ph;
   seg_physics(d, in, i, ph);
   int64_t c = cell_of(d, ph.ev, ph.plane, ph.mpx, ph.mpy);
   const bool on_plane = ph.mpx >= 0 && ph.mpx < d.nx && ph.mpy >= 0 && ph.mpy < d.ny;
   bool keep = c >= 0 && (ph.mask ? true : on_plane);
   if (status && ph.ev >= d.n_events) atomicOr(status, LSIM_STATUS_BAD_EVENT);
-  keys[i] = keep ? (uint32_t)c : (uint32_t)d.n_cells;
+  cell_key = keep ? (uint32_t)c : (uint32_t)d.n_cells;
+  const int S = d.nt - d.L - ph.k0;
+  s_key = (uint32_t)(S < 0 ? 0 : (S > d.nout + 2 ? d.nout + 2 : S));
+}
+
+// First sort stage: window keys (input order).
+__global__ void k_keys(Dev d, SegIn in, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, int* __restrict__ status) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= in.n) return;
+  uint32_t ck, sk;
+  seg_keys(d, in, i, ck, sk, status);
+  keys[i] = sk;
   vals[i] = (uint32_t)i;
+}
+
+// Second stage: the cell keys of the window-sorted order (a stable sort by cell follows).
+__global__ void k_cell_keys(Dev d, SegIn in, const uint32_t* __restrict__ vals, int64_t n, uint32_t* __restrict__ keys) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t ck, sk;
+  seg_keys(d, in, vals[j], ck, sk, nullptr);
+  keys[j] = ck;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -626,14 +648,15 @@ constexpr int kMaxCells = 17 * 17;  // (2n+1)^2 for number_pix_neighbors <= 8
 
 // The non-empty cells of a row's neighbourhood.  Cell i = (a, b), a, b in [-nn, nn] in row-major
 // order, is the segment cell of the pixel (px - a, py - b); it is listed when cls(a, b) >= 0 and
-// it holds segments, ordered by (cls, i).  One thread per cell loads the cell's segment range;
+// it holds segments, ordered by (cls, i); cnt(begin, end) gives how many of the cell's leading
+// segments to list (a cell with none is left out).  One thread per cell loads the cell's segment range;
 // each listed cell then finds its place and the number of segments before it by one pass over
 // the keys in LDS.  Writes c_beg, c_ab (a + 64 << 8 | b + 64), c_cum[0..k] (exclusive sums of the
 // cells' segment counts) and ncls[c] = number of listed cells of class <= c, c < 3.  scratch
 // holds 3 * kMaxCells ints; the call begins and ends with every thread of the block in it.
-template <int NT, class Cls>
-__device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict__ rng, int cell, Cls cls, int* scratch,
-                                           int* c_beg, int* c_cum, int* c_ab, int* ncls) {
+template <int NT, class Cls, class Cnt>
+__device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict__ rng, int cell, Cls cls, Cnt cnt,
+                                           int* scratch, int* c_beg, int* c_cum, int* c_ab, int* ncls) {
   const int t = threadIdx.x;
   const int side = 2 * d.nn + 1, ncell = side * side;
   const int px = cell % d.gx - d.halo;
@@ -649,9 +672,10 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
     int key = -1, n = 0, beg = 0;
     if (c >= 0 && qx >= -d.halo && qx < d.nx + d.halo && qy >= -d.halo && qy < d.ny + d.halo) {
       const int2 rg = rng[((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo)];
-      if (rg.y > rg.x) {
+      const int m = rg.y > rg.x ? cnt(rg.x, rg.y) : 0;
+      if (m > 0) {
         key = c * ncell + i;
-        n = rg.y - rg.x;
+        n = m;
         beg = rg.x;
       }
     }
@@ -749,7 +773,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
         return !near ? 0 : ((a == 0 && b == 0) ? 2 : 1);
       },
-      reinterpret_cast<int*>(acc), c_beg, c_cum, c_ab, ncls);
+      [](int b, int e) { return e - b; }, reinterpret_cast<int*>(acc), c_beg, c_cum, c_ab, ncls);
   for (int o = t; o < WAVES * rstride; o += NT) acc[o] = 0.f;
   __syncthreads();
 

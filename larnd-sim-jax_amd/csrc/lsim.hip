@@ -223,11 +223,13 @@ int bits_for(int64_t v) {
   return b;
 }
 
-// The sort result lands in (k0, v0) or (k1, v1) depending on the pass count; the pass count is a
-// pure function of n_cells, so both phases agree without storing anything.
-void sorted_buffers(const Ws& w, int64_t n_cells, uint32_t*& keys, uint32_t*& vals) {
-  const int bits = bits_for(n_cells);
-  const int passes = (bits + 7) / 8;
+// Segments are sorted in two stable stages, by window key (sort_bits_window) then by cell key
+// (bits_for(n_cells)); the result lands in (k0, v0) or (k1, v1) depending on the total pass count,
+// a pure function of nout and n_cells, so every phase agrees without storing anything.
+int sort_bits_window(const Dev& d) { return bits_for((int64_t)d.nout + 2); }
+
+void sorted_buffers(const Ws& w, const Dev& d, uint32_t*& keys, uint32_t*& vals) {
+  const int passes = (sort_bits_window(d) + 7) / 8 + (bits_for(d.n_cells) + 7) / 8;
   keys = (passes & 1) ? w.k1 : w.k0;
   vals = (passes & 1) ? w.v1 : w.v0;
 }
@@ -434,11 +436,17 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
     const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_keys, dim3(g), dim3(256), 0, st, d, in, w.k0, w.v0, status);
     hipError_t err;
-    const int bits = bits_for(d.n_cells);
-    radix_sort_pairs(w.k0, w.v0, w.k1, w.v1, n, bits, w.sort_scratch, st, &err);
+    const int cur = radix_sort_pairs(w.k0, w.v0, w.k1, w.v1, n, sort_bits_window(d), w.sort_scratch, st, &err);
+    HIP_TRY(err);
+    uint32_t* ka = cur ? w.k1 : w.k0;
+    uint32_t* va = cur ? w.v1 : w.v0;
+    uint32_t* kb = cur ? w.k0 : w.k1;
+    uint32_t* vb = cur ? w.v0 : w.v1;
+    hipLaunchKernelGGL(k_cell_keys, dim3(g), dim3(256), 0, st, d, in, va, n, ka);
+    radix_sort_pairs(ka, va, kb, vb, n, bits_for(d.n_cells), w.sort_scratch, st, &err);
     HIP_TRY(err);
     uint32_t *keys, *vals;
-    sorted_buffers(w, d.n_cells, keys, vals);
+    sorted_buffers(w, d, keys, vals);
     hipLaunchKernelGGL(k_records, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, w.rec, w.prs);
     hipLaunchKernelGGL(k_cells, dim3(g), dim3(256), 0, st, d, keys, n, w.rng, w.flag);
     HIP_TRY((exclusive_scan<int, int>(w.flag, w.ridx, n, w.scan_tmp, w.n_rows, st)));
@@ -462,7 +470,7 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_s
   hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, w.n_rows, out->n_rows, max_rows, out->status);
   if (n_seg > 0) {
     uint32_t *keys, *vals;
-    sorted_buffers(w, d.n_cells, keys, vals);
+    sorted_buffers(w, d, keys, vals);
     const unsigned g = (unsigned)((n_seg + 3) / 4);
     const int64_t nblk = (n_seg + 255) / 256;
     const int64_t units = (int64_t)(2 * d.nn + 1) * (2 * d.nn + 1) * nblk;
@@ -607,7 +615,7 @@ int lsim_segment_records(lsim_ctx* c, int64_t n_seg, int32_t n_events, void* ws,
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
   if (n_seg <= 0) return LSIM_OK;
   uint32_t *keys, *vals;
-  sorted_buffers(w, d.n_cells, keys, vals);
+  sorted_buffers(w, d, keys, vals);
   HIP_TRY(hipMemcpyAsync(rec_out, w.rec, (size_t)n_seg * sizeof(SegRec), hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipMemcpyAsync(keys_out, keys, (size_t)n_seg * 4, hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipMemcpyAsync(order_out, vals, (size_t)n_seg * 4, hipMemcpyDeviceToDevice, st));
@@ -652,7 +660,7 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
   Tables tb{c->TM3, c->TM, c->TN, c->TNP, c->DM, c->DN, c->CTM, c->CTN};
   SegIn in = make_segin(seg);
   uint32_t *keys, *vals;
-  sorted_buffers(w, d.n_cells, keys, vals);
+  sorted_buffers(w, d, keys, vals);
   if (max_rows > 0) {
     hipLaunchKernelGGL(k_fee_adjoint, dim3((unsigned)((max_rows + 63) / 64)), dim3(64), 0, st, d, (const FeeTape*)fwd->tape,
                        d_adc, d_z, fwd->n_rows, max_rows, pieces, dz);
