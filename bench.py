@@ -13,6 +13,7 @@ Adam fit of the 7 parameters on those 32 events (steps/s, parameter recovery), a
 baseline (the float32 NumPy restatement of the reference algorithm on a bounded sample, on the
 host cores).
 
+``--batched-bwd-only`` runs only the 32-event forward+backward graph replays (for kernel profiles).
 ``--copy-calibrate`` only runs a plain 256 MiB device copy five times: profiled with the same
 ``rocprofv3 --pmc FETCH_SIZE`` / ``WRITE_SIZE`` passes it gives the counter-to-bytes factor
 used to turn the waveform kernel's counters into HBM bytes per launch.
@@ -66,6 +67,75 @@ def cpu_baseline(params, lut, tracks, fields, n_sample: int):
                 host_cpu_count=os.cpu_count())
 
 
+def batched_fwd_bwd(simr, params, dev, steps, fd_baseline=True):
+    """32 uneven events (20k-200k segments): forward + backward replayed as one hipGraph, waveform
+    rows recomputed (not stored), against the per-event forward-then-finite-difference gradient."""
+    import numpy as np
+    import torch
+
+    from larndsim_amd import sim as S, synthetic
+
+    log("[bench] generating 32 events (20k-200k segments)")
+    ft, ff = synthetic.workload("c5")
+    NF = ft.shape[0]
+    ev = ft[:, ff.index("eventID")].astype(np.int64)
+    frun = S.Runner(simr, params, torch.from_numpy(ft).to(dev), ff, n_events=32, want_wfs=False, want_tape=True)
+    rng = np.random.default_rng(9)
+    frun.set_cotangents(torch.from_numpy(rng.normal(size=(frun.max_rows * params.MAX_ADC_VALUES,))
+                                         .astype(np.float32)).to(dev))
+    frun.forward()
+    frun.backward()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        frun.forward()
+        frun.backward()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        frun.forward()
+        frun.backward()
+    torch.cuda.synchronize()
+    for _ in range(2):
+        g.replay()
+    torch.cuda.synchronize()
+    fsteps = max(3, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(fsteps):
+        g.replay()
+    torch.cuda.synchronize()
+    dtf = (time.perf_counter() - t0) / fsteps
+    del g
+    res = {"batched_fwd_bwd": dict(events=32, segments=NF, rows=frun.rows, ms_per_step=dtf * 1e3, segments_per_s=NF / dtf,
+                                   graph=True, wfs_stored=False)}
+    del frun
+    if not fd_baseline:
+        log(f"[bench] batched fwd+bwd {NF} segments {dtf * 1e3:.2f} ms -> {NF / dtf:.3e} seg/s")
+        return res
+    # baseline: per event, 1 + 14 plain forwards (central differences, h = 1e-3 relative)
+    t_events = 0.0
+    for e in range(32):
+        sub = ft[ev == e].copy()
+        sub[:, ff.index("eventID")] = 0
+        erun = S.Runner(simr, params, torch.from_numpy(sub).to(dev), ff, n_events=1, want_wfs=True)
+        erun.forward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            erun.forward()
+        torch.cuda.synchronize()
+        t_events += (time.perf_counter() - t0) / 3
+        del erun
+    t_fd = 15 * t_events
+    res["per_event_fd_baseline"] = dict(forwards_per_event=15, ms_per_gradient=t_fd * 1e3, segments_per_s=NF / t_fd,
+                                        adjoint_speedup=t_fd / dtf)
+    log(f"[bench] batched fwd+bwd {NF} segments {dtf * 1e3:.2f} ms -> {NF / dtf:.3e} seg/s; "
+        f"per-event FD {t_fd * 1e3:.1f} ms (x{t_fd / dtf:.1f})")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +149,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--fit-steps", type=int, default=200, help="Adam steps of the 32-event gradient fit (0: skip)")
     ap.add_argument("--copy-calibrate", action="store_true")
+    ap.add_argument("--batched-bwd-only", action="store_true",
+                    help="profiling: only the 32-event batched forward+backward graph replays (no baseline)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events shard across GPUs "
@@ -102,6 +174,11 @@ def main():
     dev = torch.device("cuda")
     params = consts.default_params(number_pix_neighbors=4, RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0)
     lut = synthetic.make_lut(0)
+    if a.batched_bwd_only:
+        if a.waves:
+            S.set_row_waves(a.waves)
+        print(json.dumps(batched_fwd_bwd(S.Simulator(params, lut), params, dev, a.steps, fd_baseline=False)))
+        return
     log("[bench] generating the 200k-segment event")
     tracks, fields = synthetic.workload("c2")
     N = tracks.shape[0]
@@ -204,61 +281,7 @@ def main():
 
     # ---- batched forward + backward: 32 uneven events, one graph replay, vs per-event FD ----
     if not a.no_batched_bwd:
-        log("[bench] generating 32 events (20k-200k segments)")
-        ft, ff = synthetic.workload("c5")
-        NF = ft.shape[0]
-        ev = ft[:, ff.index("eventID")].astype(np.int64)
-        frun = S.Runner(simr, params, torch.from_numpy(ft).to(dev), ff, n_events=32, want_wfs=False, want_tape=True)
-        rng = np.random.default_rng(9)
-        frun.set_cotangents(torch.from_numpy(rng.normal(size=(frun.max_rows * params.MAX_ADC_VALUES,))
-                                             .astype(np.float32)).to(dev))
-        frun.forward()
-        frun.backward()
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            frun.forward()
-            frun.backward()
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
-        with torch.cuda.graph(g):
-            frun.forward()
-            frun.backward()
-        torch.cuda.synchronize()
-        for _ in range(2):
-            g.replay()
-        torch.cuda.synchronize()
-        fsteps = max(3, a.steps // 4)
-        t0 = time.perf_counter()
-        for _ in range(fsteps):
-            g.replay()
-        torch.cuda.synchronize()
-        dtf = (time.perf_counter() - t0) / fsteps
-        del g
-        # baseline: per event, 1 + 14 plain forwards (central differences, h = 1e-3 relative)
-        t_events = 0.0
-        for e in range(32):
-            sub = ft[ev == e].copy()
-            sub[:, ff.index("eventID")] = 0
-            erun = S.Runner(simr, params, torch.from_numpy(sub).to(dev), ff, n_events=1, want_wfs=True)
-            erun.forward()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(3):
-                erun.forward()
-            torch.cuda.synchronize()
-            t_events += (time.perf_counter() - t0) / 3
-            del erun
-        t_fd = 15 * t_events
-        extra["batched_fwd_bwd"] = dict(events=32, segments=NF, rows=frun.rows, ms_per_step=dtf * 1e3,
-                                        segments_per_s=NF / dtf, graph=True, wfs_stored=False)
-        extra["per_event_fd_baseline"] = dict(forwards_per_event=15, ms_per_gradient=t_fd * 1e3,
-                                              segments_per_s=NF / t_fd, adjoint_speedup=t_fd / dtf)
-        log(f"[bench] batched fwd+bwd {NF} segments {dtf * 1e3:.2f} ms -> {NF / dtf:.3e} seg/s; "
-            f"per-event FD {t_fd * 1e3:.1f} ms (x{t_fd / dtf:.1f})")
-        del frun
+        extra.update(batched_fwd_bwd(simr, params, dev, a.steps))
 
     # ---- gradient fit: 32 uneven events, Adam on the 7 parameters from nominal to a perturbed target ----
     if a.fit_steps > 0:
