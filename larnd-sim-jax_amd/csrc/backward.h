@@ -466,7 +466,46 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
   constexpr int NX = kAdjBinItems ? NB * NB : NB;
   constexpr int NI = NEAR ? NX + 1 : 1;
   int cur = 0;
-  for (int it = t; it < tot * NI; it += NT) {
+  if (!NEAR) {
+    // far pairs: the stripe only.  Each thread fetches its next pair's record (the first 16 bytes
+    // of the pair record) and chain factors while the current pair's table loads are in flight.
+    struct FarItem {
+      int4 P;  // S, ne, f, flags
+      double fN0, fN1, fN2, fN3, fF;
+      int ab;
+    };
+    auto fetch = [&](int i, FarItem& x) {
+      while (i >= c_cum[cur + 1]) ++cur;
+      const int sg = c_beg[cur] + (i - c_cum[cur]);
+      x.ab = c_ab[cur];
+      x.P = *reinterpret_cast<const int4*>(prs + sg);
+      const AdjRec* A = adj + sg;
+      x.fN0 = A->fN[0];
+      x.fN1 = A->fN[1];
+      x.fN2 = A->fN[2];
+      x.fN3 = A->fN[3];
+      x.fF = A->fF;
+    };
+    FarItem x, y;
+    if (t < tot) fetch(t, x);
+    for (int it = t; it < tot; it += NT) {
+      if (it + NT < tot) fetch(it + NT, y);
+      const float ne = __int_as_float(x.P.y);
+      if (ne != 0.f) {  // every listed pair is live (G does not vanish before its window)
+        PairCtx pc;
+        pair_ctx(d, x.P.x, ne, __int_as_float(x.P.z), G, pc);
+        double gN = 0.0, gF = 0.0;
+        stripe_adjoint(d, tb, (x.P.w >> 8) & 255, (x.P.w >> 16) & 255, pc, (x.ab >> 8) - 64, (x.ab & 255) - 64, G, gN,
+                       gF);
+        g[0] += gN * x.fN0;
+        g[1] += gN * x.fN1;
+        g[2] += gN * x.fN2 + gF * x.fF;
+        g[3] += gN * x.fN3;
+      }
+      x = y;
+    }
+  }
+  for (int it = t; NEAR && it < tot * NI; it += NT) {
     const int i = it / NI, part = it - i * NI;
     while (i >= c_cum[cur + 1]) ++cur;
     const int s = c_beg[cur] + (i - c_cum[cur]);
@@ -474,19 +513,6 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
     const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
     double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
     PairCtx pc;
-    if (!NEAR) {
-      // the 32-byte pair record: window start first (most far pairs end where G has vanished)
-      const PairRec* P = prs + s;
-      const int S = P->S;
-      if (pair_dead(S, G)) continue;
-      const float ne = P->ne;
-      if (ne == 0.f) continue;
-      pair_ctx(d, S, ne, P->f, G, pc);
-      const int fl = P->flags;
-      stripe_adjoint(d, tb, (fl >> 8) & 255, (fl >> 16) & 255, pc, a, b, G, gN, gF);
-      add_chain(adj + s, gN, gF, gS, gL, false, g);
-      continue;
-    }
     const SegRec R = rec[s];
     if (R.ne == 0.f) continue;
     const int S = d.nt - d.L - R.k0;
