@@ -306,7 +306,7 @@ __device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, i
   const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
   double y1, y0;
   g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, c, L, &y1, &y0);
-  const float2 D = tb.DN[sidx * d.nt + (d.nt - L - c.S)];  // k0 = nt - L - S
+  const float2 D = tb.DN[(int64_t)(d.nt - L - c.S) * d.lx * d.ly + sidx];  // tick-major, k0 = nt - L - S
   const double corr = (double)D.x - c.f * (double)D.y;
   gN += (1.0 - c.f) * y1 + c.f * y0 + c.gc * corr;
   gF += c.ne * (y0 - y1 + (c.gc1 - c.gc0) * corr - c.gc * (double)D.y);
@@ -351,7 +351,7 @@ __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, con
     const int64_t cell = ((int64_t)R.idx * cm + cx) * cm + cyv[jj];
     double y1[3], y0[3];
     g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c, L, y1, y0);
-    const float2 D = tb.DM[cell * d.nt + R.k0];
+    const float2 D = tb.DM[(int64_t)R.k0 * d.nm * cm * cm + cell];  // tick-major
     const double corr = (double)D.x - f * (double)D.y;
     const double wxy = (double)WXi * (double)WY[jj];
     const double dwxy = (double)DWXi * (double)WY[jj] + (double)WXi * (double)DWY[jj];

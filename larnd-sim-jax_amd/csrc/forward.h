@@ -89,8 +89,59 @@ __global__ void k_cell_keys(Dev d, SegIn in, const uint32_t* __restrict__ vals, 
 // 2. records in cell order, cell ranges, pixel rows
 // ------------------------------------------------------------------------------------------
 
+// Boundary corrections of every (row offset, segment) pair of record R, written where the record
+// is built: dtot[ab][i] = n_e (D.x - f D.y) of the neighbour stripe (raw response, unless the offset
+// is (0, 0)) plus, for |a|, |b| <= 1, the sum over the diffusion bins landing on that row of
+// n_bin (D.x - f D.y) of the idx template (sim_jax.py:227-261), D = (C[nt-L] - C[k0],
+// C[k0+1] - C[k0]), ab = (a + n)(2n + 1) + (b + n).  The D tables are tick-major, so all of one
+// segment's entries lie in the tick-k0 plane; the waveform kernels then read dtot[ab][cell range]
+// contiguously.
+template <int NB>
+__device__ __forceinline__ void seg_corrections(const Dev& d, const float2* __restrict__ DN, const float2* __restrict__ DM,
+                                                const SegRec& R, int ux, int uy, int64_t i, int64_t n,
+                                                float* __restrict__ dtot) {
+  constexpr int SYM = (NB - 1) / 2;
+  const int W = 2 * d.nn + 1, nsb = d.nsb, cm = d.cm;
+  const float f = R.f;
+  const float2* DNk = DN + (int64_t)R.k0 * d.lx * d.ly;
+  const float2* DMk = DM + ((int64_t)R.k0 * d.nm + R.idx) * cm * cm;
+  for (int ab = 0; ab < W * W; ++ab) {
+    const int a = ab / W - d.nn, b = ab % W - d.nn;
+    float delta = 0.f;
+    if (a != 0 || b != 0) {
+      const float2 D = DNk[offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb)];
+      delta = R.ne * (D.x - f * D.y);
+    }
+    if (a >= -1 && a <= 1 && b >= -1 && b <= 1) {
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const int vx = ux + k - SYM;
+        const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
+        if (dx != a) continue;
+        const float wxs = R.ne * R.wx[k];
+        const int cx = offset_bin(vx - dx * nsb, 0, nsb);
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) {
+          const int vy = uy + jj - SYM;
+          const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
+          if (dy != b) continue;
+          const float c = wxs * R.wy[jj];
+          if (c != 0.f) {
+            const float2 D = DMk[cx * cm + offset_bin(vy - dy * nsb, 0, nsb)];
+            delta += c * (D.x - f * D.y);
+          }
+        }
+      }
+    }
+    dtot[(int64_t)ab * n + i] = delta;
+  }
+}
+
+// One thread per segment in sorted order: the records and the boundary corrections.
+template <int NB>
 __global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
-                          SegRec* __restrict__ rec, PairRec* __restrict__ prs) {
+                          const float2* __restrict__ DN, const float2* __restrict__ DM, SegRec* __restrict__ rec,
+                          PairRec* __restrict__ prs, float* __restrict__ dtot) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || keys[i] >= (uint32_t)d.n_cells) return;
   Phys ph;
@@ -126,6 +177,7 @@ __global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, co
   P.U0 = offset_bin(ux, 0, d.nsb) * d.ly * kWinPad * 4;
   P.V0 = offset_bin(uy, 0, d.nsb) * kWinPad * 4;
   prs[i] = P;
+  seg_corrections<NB>(d, DN, DM, r, ux, uy, i, n, dtot);
 }
 
 // Cell ranges [begin, end) in the sorted order and the "first segment of an on-plane cell" flag
@@ -175,8 +227,8 @@ struct Tables {
   const float* TM;    // [nm][cm][cm][L]   main-bin template windows
   const float* TN;    // [lx][ly][L]       raw response window (neighbours)
   const float* TNP;   // [lx][ly][kWinPad] the same, padded
-  const float2* DM;   // [nm][cm][cm][nt]  (C[nt-L]-C[k], C[min(k+1,nt-1)]-C[k]) of template m
-  const float2* DN;   // [lx][ly][nt]      same for the raw response
+  const float2* DM;   // [nt][nm][cm][cm]  (C[nt-L]-C[k], C[min(k+1,nt-1)]-C[k]) of template m (tick-major)
+  const float2* DN;   // [nt][lx][ly]      same for the raw response
   const double* CTM;  // [nm][cm][cm][L+1] window prefix sums (adjoint)
   const double* CTN;  // [lx][ly][L+1]
 };
@@ -565,65 +617,6 @@ __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* 
   o[0] = v1;
   o[kWave] = v2;
   o[2 * kWave] = v3;
-}
-
-// Boundary corrections of every (segment, row offset) pair, one thread per pair:
-// dtot[ab][s] = n_e (D.x - f D.y) of the neighbour stripe (raw response, unless the offset is
-// (0, 0)) plus, for |a|, |b| <= 1, sum over the diffusion bins landing on that row of
-// n_bin (D.x - f D.y) of the idx template (sim_jax.py:227-261), D = (C[nt-L] - C[k0],
-// C[k0+1] - C[k0]).  ab = (a + n)(2n + 1) + (b + n).  Work units (ab, block of 256 segments)
-// run ab-major in contiguous XCD bands, so the D table rows one offset reads (100 sub-bin
-// positions x nt) stay in one XCD's L2; the waveform kernels then read dtot[ab][cell range]
-// contiguously instead of gathering D per pair.
-template <int NB>
-__global__ __launch_bounds__(256) void k_corr(Dev d, Tables tb, const SegRec* __restrict__ rec,
-                                              const PairRec* __restrict__ prs, const uint32_t* __restrict__ keys,
-                                              int64_t n, int64_t nblk, float* __restrict__ dtot) {
-  constexpr int SYM = (NB - 1) / 2;
-  const int W = 2 * d.nn + 1;
-  const int64_t units = (int64_t)W * W * nblk;
-  const int64_t xcd = blockIdx.x % kXcd, j = blockIdx.x / kXcd;
-  const int64_t q = units / kXcd, rem = units % kXcd;
-  if (j >= q + (xcd < rem ? 1 : 0)) return;
-  const int64_t u = xcd * q + (xcd < rem ? xcd : rem) + j;
-  const int ab = (int)(u / nblk);
-  const int64_t s = (u % nblk) * 256 + threadIdx.x;
-  if (s >= n || keys[s] >= (uint32_t)d.n_cells) return;
-  const int a = ab / W - d.nn, b = ab % W - d.nn;
-  const int nsb = d.nsb, cm = d.cm;
-  const PairRec F = prs[s];  // 32 bytes per segment: the whole pass re-reads 32 B, not 80 B, per offset
-  const int ux = (F.flags >> 8) & 0xff, uy = (F.flags >> 16) & 0xff;
-  const int k0 = d.nt - d.L - F.S;
-  const float f = F.f;
-  float delta = 0.f;
-  if (a != 0 || b != 0) {
-    const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
-    const float2 D = tb.DN[sidx * d.nt + k0];
-    delta = F.ne * (D.x - f * D.y);
-  }
-  if (a >= -1 && a <= 1 && b >= -1 && b <= 1) {
-    const SegRec R = rec[s];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      const int vx = ux + i - SYM;
-      const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
-      if (dx != a) continue;
-      const float wxs = R.ne * R.wx[i];
-      const int cx = offset_bin(vx - dx * nsb, 0, nsb);
-#pragma unroll
-      for (int jj = 0; jj < NB; ++jj) {
-        const int vy = uy + jj - SYM;
-        const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
-        if (dy != b) continue;
-        const float c = wxs * R.wy[jj];
-        if (c != 0.f) {
-          const float2 D = tb.DM[(((int64_t)R.idx * cm + cx) * cm + offset_bin(vy - dy * nsb, 0, nsb)) * d.nt + k0];
-          delta += c * (D.x - f * D.y);
-        }
-      }
-    }
-  }
-  dtot[(int64_t)ab * n + s] = delta;
 }
 
 // Center pair from the precomputed window: V = vc[s] (charge included), Delta = dtc[s].
@@ -1044,7 +1037,7 @@ __global__ __launch_bounds__(256) void k_lut_tables(Dev d, const float* __restri
   for (int k = threadIdx.x; k < nt; k += blockDim.x) {
     const double c0 = T[k];
     const double c1 = T[k + 1 < nt ? k + 1 : nt - 1];
-    Dt[sbase * nt + k] = make_float2((float)(cref - c0), (float)(c1 - c0));
+    Dt[(int64_t)k * gridDim.x + sbase] = make_float2((float)(cref - c0), (float)(c1 - c0));  // tick-major
   }
   if (threadIdx.x == 0) {
     double c = 0.0;

@@ -447,7 +447,13 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
     HIP_TRY(err);
     uint32_t *keys, *vals;
     sorted_buffers(w, d, keys, vals);
-    hipLaunchKernelGGL(k_records, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, w.rec, w.prs);
+    if (d.nb == 5) {
+      hipLaunchKernelGGL(k_records<5>, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, c->DN, c->DM, w.rec, w.prs, w.dtot);
+    } else if (d.nb == 3) {
+      hipLaunchKernelGGL(k_records<3>, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, c->DN, c->DM, w.rec, w.prs, w.dtot);
+    } else {
+      hipLaunchKernelGGL(k_records<1>, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, c->DN, c->DM, w.rec, w.prs, w.dtot);
+    }
     hipLaunchKernelGGL(k_cells, dim3(g), dim3(256), 0, st, d, keys, n, w.rng, w.flag);
     HIP_TRY((exclusive_scan<int, int>(w.flag, w.ridx, n, w.scan_tmp, w.n_rows, st)));
     hipLaunchKernelGGL(k_rows_map, dim3(g), dim3(256), 0, st, d, keys, w.flag, w.ridx, n, w.row_cell, w.cell_row);
@@ -472,18 +478,12 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_s
     uint32_t *keys, *vals;
     sorted_buffers(w, d, keys, vals);
     const unsigned g = (unsigned)((n_seg + 3) / 4);
-    const int64_t nblk = (n_seg + 255) / 256;
-    const int64_t units = (int64_t)(2 * d.nn + 1) * (2 * d.nn + 1) * nblk;
-    const unsigned gc = (unsigned)(((units + kXcd - 1) / kXcd) * kXcd);
     if (d.nb == 5) {
       hipLaunchKernelGGL(k_center<5>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-      hipLaunchKernelGGL(k_corr<5>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, w.prs, keys, n_seg, nblk, w.dtot);
     } else if (d.nb == 3) {
       hipLaunchKernelGGL(k_center<3>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-      hipLaunchKernelGGL(k_corr<3>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, w.prs, keys, n_seg, nblk, w.dtot);
     } else {
       hipLaunchKernelGGL(k_center<1>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-      hipLaunchKernelGGL(k_corr<1>, dim3(gc), dim3(256), 0, st, d, tb, w.rec, w.prs, keys, n_seg, nblk, w.dtot);
     }
   }
   if (max_rows > 0) {
