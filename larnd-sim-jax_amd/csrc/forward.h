@@ -287,11 +287,12 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
   const int nout = d.nout;
   const int lane = t & (kWave - 1), wid = t / kWave;
   const int H = d.H;
+  // ticks interleaved over the threads (o = t + NT u): a wave's lanes read consecutive doubles
   for (int k = 0; k < H; ++k) {
     int cand = 0x7fffffff;
 #pragma unroll
     for (int u = 0; u < TPER; ++u) {
-      const int o = t * TPER + u;
+      const int o = t + NT * u;
       if (o <= nout - 2) {
         if (Q[o + 1] >= d.thr && Q[o] <= d.thr && o < cand) cand = o;
       }
@@ -316,7 +317,7 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < TPER; ++u) {
-      const int o = t * TPER + u;
+      const int o = t + NT * u;
       if (o < nout) {
         const double x = Q[o] - Qg;
         Q[o] = x < 0 ? 0.0 : x;
