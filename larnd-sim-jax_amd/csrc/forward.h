@@ -659,6 +659,10 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
   int* s_key = scratch;
   int* s_cnt = scratch + kMaxCells;
   int* s_beg = scratch + 2 * kMaxCells;
+  if (t == 0) {
+    c_cum[0] = 0;
+    ncls[0] = ncls[1] = ncls[2] = 0;
+  }
   for (int i = t; i < ncell; i += NT) {
     const int a = i / side - d.nn, b = i % side - d.nn;
     const int c = cls(a, b);
@@ -678,37 +682,36 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
     s_beg[i] = beg;
   }
   __syncthreads();
+  // each listed cell: its rank and the segments before it; the last cell of class 0, of classes
+  // <= 1 and of all classes writes that count (the totals stay 0 when there is no such cell)
   for (int i = t; i < ncell; i += NT) {
     const int key = s_key[i];
     if (key < 0) continue;
-    int rank = 0, before = 0;
+    const int c = key / ncell;
+    int rank = 0, before = 0, after0 = 0, after1 = 0, after = 0;
     for (int j = 0; j < ncell; ++j) {
       const int kj = s_key[j];
-      if (kj >= 0 && kj < key) {
+      if (kj < 0) continue;
+      if (kj < key) {
         ++rank;
         before += s_cnt[j];
+      } else if (kj > key) {
+        const int cj = kj / ncell;
+        after0 += cj == 0 ? 1 : 0;
+        after1 += cj <= 1 ? 1 : 0;
+        ++after;
       }
     }
     const int a = i / side - d.nn, b = i % side - d.nn;
     c_beg[rank] = s_beg[i];
     c_cum[rank] = before;
     c_ab[rank] = ((a + 64) << 8) | (b + 64);
-  }
-  if (t == 0) {
-    int k = 0, tot = 0, n0 = 0, n1 = 0;
-    for (int j = 0; j < ncell; ++j) {
-      const int kj = s_key[j];
-      if (kj < 0) continue;
-      const int c = kj / ncell;
-      ++k;
-      tot += s_cnt[j];
-      n0 += c == 0 ? 1 : 0;
-      n1 += c <= 1 ? 1 : 0;
+    if (c == 0 && after0 == 0) ncls[0] = rank + 1;
+    if (c <= 1 && after1 == 0) ncls[1] = rank + 1;
+    if (after == 0) {
+      ncls[2] = rank + 1;
+      c_cum[rank + 1] = before + s_cnt[i];
     }
-    c_cum[k] = tot;
-    ncls[0] = n0;
-    ncls[1] = n1;
-    ncls[2] = k;
   }
   __syncthreads();
 }
