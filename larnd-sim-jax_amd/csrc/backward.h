@@ -24,85 +24,122 @@ constexpr int kNGrad = 7;
 #endif
 constexpr int kAdjWaves = LSIM_ADJ_WAVES;  // waves per pixel-row workgroup of the adjoint
 
-__global__ void k_fee_adjoint(Dev d, const FeeTape* __restrict__ tape, const float* __restrict__ d_adc,
-                              const float* __restrict__ d_z, const int64_t* __restrict__ n_rows_dev, int64_t max_rows,
-                              Pieces* __restrict__ pieces, double* __restrict__ dz_part) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ double readlane_d(double v, int j) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+  return __hiloint2double(hi, lo);
+}
+
+// One wave per row (kFeeRows rows per workgroup).  Lane j < 2H follows tracked point j: the reverse
+// pass over the H self-trigger passes keeps A[j] = d loss / d Q at point j in lane j, each pass's
+// masked sum taken in point order; the points are then merged by tick (lane-parallel: the first
+// lane of a tick owns it and sums the lanes of that tick in lane order), and lane 0 turns the
+// per-tick jumps into the step function (suffix sums, equal neighbours merged, zero tail dropped).
+constexpr int kFeeRows = 4;
+
+__global__ __launch_bounds__(kFeeRows * kWave) void k_fee_adjoint(Dev d, const FeeTape* __restrict__ tape,
+                                                                  const float* __restrict__ d_adc,
+                                                                  const float* __restrict__ d_z,
+                                                                  const int64_t* __restrict__ n_rows_dev, int64_t max_rows,
+                                                                  Pieces* __restrict__ pieces, double* __restrict__ dz_part) {
+  __shared__ int spos[kFeeRows][2 * kMaxH];
+  __shared__ double sval[kFeeRows][2 * kMaxH];
+  __shared__ int sx[kFeeRows][2 * kMaxH];
+  __shared__ float sc[kFeeRows][2 * kMaxH];
+  __shared__ int sn[kFeeRows];
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int64_t r = (int64_t)blockIdx.x * kFeeRows + wv;
   int64_t P = *n_rows_dev;
   if (P > max_rows) P = max_rows;
   if (r >= max_rows) return;
   if (r >= P) {
-    dz_part[r] = 0.0;
+    if (lane == 0) dz_part[r] = 0.0;
     return;
   }
-  const FeeTape tp = tape[r];
-  const int H = d.H;
-  double A[2 * kMaxH];
-  for (int j = 0; j < 2 * H; ++j) A[j] = 0.0;
+  const FeeTape* tp = tape + r;
+  const int H = d.H, np2 = 2 * H;
+  const bool act = lane < np2;
+  double A = 0.0;
   for (int k = H - 1; k >= 0; --k) {
-    const uint32_t m = tp.mask[k];
+    const uint32_t m = tp->mask[k];
     double s = 0.0;
-    for (int j = 0; j < 2 * H; ++j) {
-      if ((m >> j) & 1u) s += A[j];
-      else A[j] = 0.0;
+    for (int j = 0; j < np2; ++j) {
+      const double aj = readlane_d(A, j);
+      if ((m >> j) & 1u) s += aj;
     }
-    A[H + k] -= s;
-    A[k] += (double)d_adc[r * H + k] * (double)tp.slope[k];
+    if (act && !((m >> lane) & 1u)) A = 0.0;
+    if (lane == H + k) A -= s;
+    if (lane == k) A += (double)d_adc[r * H + k] * (double)tp->slope[k];
   }
-  // merge points by position (ascending), suffix sums -> step values
-  int pos[2 * kMaxH];
-  double val[2 * kMaxH];
-  int n = 0;
-  for (int j = 0; j < 2 * H; ++j) {
-    const int p = tp.pt[j];
-    int q = 0;
-    while (q < n && pos[q] < p) ++q;
-    if (q < n && pos[q] == p) {
-      val[q] += A[j];
-    } else {
-      for (int u = n; u > q; --u) {
-        pos[u] = pos[u - 1];
-        val[u] = val[u - 1];
+  // merge the points by tick
+  const int p = act ? (int)tp->pt[lane] : 0x7fffffff;
+  bool owner = act;
+  double val = 0.0;
+  for (int j = 0; j < np2; ++j) {
+    const int pj = __builtin_amdgcn_readlane(p, j);
+    const double aj = readlane_d(A, j);
+    if (act && pj == p) {
+      val += aj;
+      if (j < lane) owner = false;
+    }
+  }
+  int rank = 0;
+  for (int j = 0; j < np2; ++j) {
+    const int pj = __builtin_amdgcn_readlane(p, j);
+    const int oj = __builtin_amdgcn_readlane(owner ? 1 : 0, j);
+    if (oj && pj < p) ++rank;
+  }
+  if (owner) {
+    spos[wv][rank] = p;
+    sval[wv][rank] = val;
+  }
+  const int n = __popcll(__ballot(owner));
+  // wave-local LDS: stores and loads of one wave complete in program order; the wave barriers
+  // keep the compiler from moving them across
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) {
+    // step values G on (pos[q-1], pos[q]]: suffix sums of the jumps
+    double suf = 0.0;
+    for (int q = n - 1; q >= 0; --q) {
+      suf += sval[wv][q];
+      sc[wv][q] = (float)(suf * d.ts);
+    }
+    // keep only the steps where G changes, then drop a zero tail
+    int m = 0;
+    float last = 0.f;
+    for (int q = 0; q < n; ++q) {
+      const float cq = sc[wv][q];
+      if (m > 0 && cq == last) {
+        sx[wv][m - 1] = spos[wv][q];  // same value: extend the previous piece
+      } else {
+        sx[wv][m] = spos[wv][q];
+        sc[wv][m] = cq;
+        last = cq;
+        ++m;
       }
-      pos[q] = p;
-      val[q] = A[j];
-      ++n;
     }
-  }
-  // step values G on (pos[q-1], pos[q]]; keep only the steps where G changes, drop a zero tail
-  float cv[2 * kMaxH];
-  double suf = 0.0;
-  for (int q = n - 1; q >= 0; --q) {
-    suf += val[q];
-    cv[q] = (float)(suf * d.ts);
-  }
-  Pieces pc;
-  int m = 0;
-  for (int q = 0; q < n; ++q) {
-    if (m > 0 && cv[q] == pc.c[m - 1]) {
-      pc.x[m - 1] = pos[q];  // same value: extend the previous piece
-    } else {
-      pc.x[m] = pos[q];
-      pc.c[m] = cv[q];
-      ++m;
+    while (m > 0 && sc[wv][m - 1] == 0.f) --m;
+    sn[wv] = m;
+    double dz = 0.0;
+    if (d_z) {
+      const int pl = tp->plane;
+      const double za = d.tpc[pl][2][0], zc = d.tpc[pl][2][1];
+      const double sgn = (zc - za) > 0 ? 1.0 : ((zc - za) < 0 ? -1.0 : 0.0);
+      for (int k = 0; k < H; ++k) dz += (double)d_z[r * H + k] * (double)tp->tick[k] * d.ts * sgn;
     }
+    dz_part[r] = dz;  // d loss / d vdrift_eff through the hit z positions
   }
-  while (m > 0 && pc.c[m - 1] == 0.f) --m;
-  pc.n = m;
-  pc.pad[0] = pc.pad[1] = pc.pad[2] = 0;
-  for (int q = m; q < 2 * kMaxH; ++q) {
-    pc.x[q] = 0x7fffffff;
-    pc.c[q] = 0.f;
+  __builtin_amdgcn_wave_barrier();
+  const int mm = __builtin_amdgcn_readfirstlane(sn[wv]);
+  Pieces* pc = pieces + r;
+  if (lane == 0) {
+    pc->n = mm;
+    pc->pad[0] = pc->pad[1] = pc->pad[2] = 0;
   }
-  pieces[r] = pc;
-  double dz = 0.0;
-  if (d_z) {
-    const int pl = tp.plane;
-    const double za = d.tpc[pl][2][0], zc = d.tpc[pl][2][1];
-    const double sgn = (zc - za) > 0 ? 1.0 : ((zc - za) < 0 ? -1.0 : 0.0);
-    for (int k = 0; k < H; ++k) dz += (double)d_z[r * H + k] * (double)tp.tick[k] * d.ts * sgn;
+  if (lane < 2 * kMaxH) {
+    pc->x[lane] = lane < mm ? sx[wv][lane] : 0x7fffffff;
+    pc->c[lane] = lane < mm ? sc[wv][lane] : 0.f;
   }
-  dz_part[r] = dz;  // d loss / d vdrift_eff through the hit z positions
 }
 
 // Per-segment factors of the chain rule, in the same (sorted) order as the SegRec records.

@@ -662,7 +662,8 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
   uint32_t *keys, *vals;
   sorted_buffers(w, d, keys, vals);
   if (max_rows > 0) {
-    hipLaunchKernelGGL(k_fee_adjoint, dim3((unsigned)((max_rows + 63) / 64)), dim3(64), 0, st, d, (const FeeTape*)fwd->tape,
+    hipLaunchKernelGGL(k_fee_adjoint, dim3((unsigned)((max_rows + kFeeRows - 1) / kFeeRows)), dim3(kFeeRows * kWave), 0, st,
+                       d, (const FeeTape*)fwd->tape,
                        d_adc, d_z, fwd->n_rows, max_rows, pieces, dz);
     if (n > 0)
       hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, in, keys, vals, n, adj);
