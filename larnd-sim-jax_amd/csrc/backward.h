@@ -234,9 +234,9 @@ __device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__
       y1[k] = gh1 * cl;
       y0[k] = gh0 * cl;
     }
-    for (int q = G.qlut[lo0]; q < G.np; ++q) {
+    const int q1 = G.qlut[hi1];  // breakpoints x_q < hi1
+    for (int q = G.qlut[lo0]; q < q1; ++q) {
       const int xq = G.x[q];
-      if (xq >= hi1) break;
       const double dq = (double)G.dc[q];
       const int i = xq - S + 1;  // 0 <= i <= L - 1
 #pragma unroll
@@ -263,9 +263,9 @@ __device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__
   }
   if (!w1 && !w0) return;
   const int lo = lo0 < lo1 ? lo0 : lo1;
-  for (int q = G.qlut[lo]; q < G.np; ++q) {
+  const int q1 = G.qlut[hi1 > 0 ? hi1 : 0];  // breakpoints x_q < hi1 (hi0 <= hi1)
+  for (int q = G.qlut[lo]; q < q1; ++q) {
     const int xq = G.x[q];
-    if (xq >= hi1) break;  // hi0 <= hi1
     const double dq = (double)G.dc[q];
     if (w1 && xq >= lo1) {
 #pragma unroll
