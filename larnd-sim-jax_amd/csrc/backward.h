@@ -59,9 +59,20 @@ __global__ __launch_bounds__(kFeeRows * kWave) void k_fee_adjoint(Dev d, const F
   const FeeTape* tp = tape + r;
   const int H = d.H, np2 = 2 * H;
   const bool act = lane < np2;
+  // everything the passes read, loaded at once: lane k < H holds pass k's mask and ADC term
+  const int p = act ? (int)tp->pt[lane] : 0x7fffffff;
+  const int mk = lane < H ? (int)tp->mask[lane] : 0;
+  const double gk = lane < H ? (double)d_adc[r * H + lane] * (double)tp->slope[lane] : 0.0;
+  double zk = 0.0;
+  if (d_z && lane < H) {
+    const int pl = tp->plane;
+    const double za = d.tpc[pl][2][0], zc = d.tpc[pl][2][1];
+    const double sgn = (zc - za) > 0 ? 1.0 : ((zc - za) < 0 ? -1.0 : 0.0);
+    zk = (double)d_z[r * H + lane] * (double)tp->tick[lane] * d.ts * sgn;
+  }
   double A = 0.0;
   for (int k = H - 1; k >= 0; --k) {
-    const uint32_t m = tp->mask[k];
+    const uint32_t m = (uint32_t)__builtin_amdgcn_readlane(mk, k);
     double s = 0.0;
     for (int j = 0; j < np2; ++j) {
       const double aj = readlane_d(A, j);
@@ -69,10 +80,11 @@ __global__ __launch_bounds__(kFeeRows * kWave) void k_fee_adjoint(Dev d, const F
     }
     if (act && !((m >> lane) & 1u)) A = 0.0;
     if (lane == H + k) A -= s;
-    if (lane == k) A += (double)d_adc[r * H + k] * (double)tp->slope[k];
+    if (lane == k) A += gk;
   }
+  double dz = 0.0;  // d loss / d vdrift_eff through the hit z positions, summed in pass order
+  for (int k = 0; k < H; ++k) dz += readlane_d(zk, k);
   // merge the points by tick
-  const int p = act ? (int)tp->pt[lane] : 0x7fffffff;
   bool owner = act;
   double val = 0.0;
   for (int j = 0; j < np2; ++j) {
@@ -120,14 +132,7 @@ __global__ __launch_bounds__(kFeeRows * kWave) void k_fee_adjoint(Dev d, const F
     }
     while (m > 0 && sc[wv][m - 1] == 0.f) --m;
     sn[wv] = m;
-    double dz = 0.0;
-    if (d_z) {
-      const int pl = tp->plane;
-      const double za = d.tpc[pl][2][0], zc = d.tpc[pl][2][1];
-      const double sgn = (zc - za) > 0 ? 1.0 : ((zc - za) < 0 ? -1.0 : 0.0);
-      for (int k = 0; k < H; ++k) dz += (double)d_z[r * H + k] * (double)tp->tick[k] * d.ts * sgn;
-    }
-    dz_part[r] = dz;  // d loss / d vdrift_eff through the hit z positions
+    dz_part[r] = dz;
   }
   __builtin_amdgcn_wave_barrier();
   const int mm = __builtin_amdgcn_readfirstlane(sn[wv]);
