@@ -55,6 +55,23 @@ def test_segment_records(sim, gpu_c1, oracle_c1):
     np.testing.assert_allclose(r["wy"], q["wy"][o], atol=1e-7)
 
 
+def test_segment_order(sim, params4, gpu_c1):
+    """Records are sorted by cell, then by the first output tick S = nt - L - k0 of their windows
+    (clamped to [0, nout + 2]), then by input index: the adjoint relies on the window order to list
+    only each cell's live prefix."""
+    rec, keys, order = segment_records(sim, gpu_c1)
+    n_cells = gpu_c1.n_events * 2 * (140 + 4) * (280 + 4)
+    valid = keys < n_cells
+    assert valid.any()
+    k, o = keys[valid].astype(np.int64), order[valid].astype(np.int64)
+    nt, L, nout = 1950, int(params4.signal_length), 2000
+    S = np.clip(nt - L - rec["k0"][valid].astype(np.int64), 0, nout + 2)
+    lex = np.lexsort((o, S, k))  # primary key last: cell, then S, then input index
+    assert np.array_equal(lex, np.arange(len(k)))
+    # every dropped segment sorts after the valid ones
+    assert not valid[np.argmax(~valid):].any() if (~valid).any() else True
+
+
 def test_rows_and_waveforms(gpu_c1, oracle_c1):
     U = oracle_c1["unique_pixels"]
     valid = U >= 0
