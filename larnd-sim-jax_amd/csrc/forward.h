@@ -466,6 +466,52 @@ __device__ __forceinline__ void store_window(float* accw, int lane, float v0, fl
   }
 }
 
+// lane i receives lane i+1's value, lane 63 receives 0 (DPP wave shift; bound_ctrl supplies the
+// zero, so no register needs clearing first)
+__device__ __forceinline__ float lane_next0(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130 /* wave_shl:1 */, 0xf, 0xf, true));
+}
+
+// Store of a neighbour-stripe window with the pair's factors precomputed: so = n_e (1-f),
+// sf = n_e f, fD = f Delta, omfD = (1-f) Delta.  v0..v2 come from a padded table row (zero past the
+// window), so interior stores need no mask beyond lane < 63.  base = this wave's partial row - 2.
+// m0, m1: 1 on lane 0 (resp. 1), else 0; fD m0 + omfD m1 is exactly the correction of the lane.
+__device__ __forceinline__ void store_stripe(float* base, int lane, float m0, float m1, float v0, float v1, float v2,
+                                             float so, float sf, float fD, float omfD, int S, int L, int nout,
+                                             bool interior) {
+  const float n0 = lane_next0(v0), n1 = lane_next0(v1), n2 = lane_next0(v2);
+  if (interior) {
+    float* p = base + S + lane;
+    float c0 = so * v0 + sf * n0;
+    c0 += fD * m0 + omfD * m1;
+    const float c1 = so * v1 + sf * n1;
+    const float c2 = so * v2 + sf * n2;
+    if (lane < kChunk) {
+      const float a0 = p[0], a1 = p[kChunk], a2 = p[2 * kChunk];
+      p[0] = a0 + c0;
+      p[kChunk] = a1 + c1;
+      p[2 * kChunk] = a2 + c2;
+    }
+  } else {
+    float* accw = base + 2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float v = c == 0 ? v0 : (c == 1 ? v1 : v2), vn = c == 0 ? n0 : (c == 1 ? n1 : n2);
+      const int m = lane - 2 + c * kChunk;
+      const int o = S + m;
+      if (lane < kChunk && m < L && o >= 0 && o < nout) {
+        float val = 0.f;
+        if (o >= 1) val = so * v + sf * vn;
+        if (o <= nout - 2) {
+          if (m == -1) val += omfD;
+          if (m == -2) val += fD;
+        }
+        accw[o] += val;
+      }
+    }
+  }
+}
+
 // Neighbour stripe only (|a| or |b| > 1): raw response window, undiffused charge.  L <= 187.
 // The loads (`far_load`) are issued for a group of pairs before any of their stores, so a wave
 // keeps several independent pairs in flight.
@@ -790,7 +836,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
       const int p = b0 + lane;
       // padding lanes: an interior window at S = 2 with zero charge and correction (adds +0)
       int rS = (2 << 1) | 1, roff = 0, rsg = 0, rab = 0;
-      float rne = 0.f, rf = 0.f, rD = 0.f;
+      float rso = 0.f, rsf = 0.f, rfD = 0.f, romfD = 0.f;
       bool bins = false;
       if (p < nfar) {
         int lo = 0, hi = nsc - 1;  // the cell holding p: last c with c_cum[c] <= p
@@ -811,36 +857,42 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         else if (b < 0) { kc += (nsb * -b - nsb / 2) * W4; mb = 1; }
         else zb = 1;
         const PairRec P = prs[sg];
-        rD = dtot[(int64_t)((a + d.nn) * side + (b + d.nn)) * nseg + sg];
+        const float D = dtot[(int64_t)((a + d.nn) * side + (b + d.nn)) * nseg + sg];
         roff = kc + ma * P.U + za * P.U0 + mb * P.V + zb * P.V0;
         rS = (P.S << 1) | (P.flags & 1);
-        rne = P.ne;
-        rf = P.f;
+        const float omf = 1.f - P.f;
+        rso = P.ne * omf;
+        rsf = P.ne * P.f;
+        rfD = P.f * D;
+        romfD = omf * D;
         rsg = sg;
         rab = ab;
         bins = a >= -1 && a <= 1 && b >= -1 && b <= 1 && P.ne != 0.f && bins_reach<NB>((P.flags >> 8) & 255, a, nsb) &&
                bins_reach<NB>((P.flags >> 16) & 255, b, nsb);
       }
       const int n = __builtin_amdgcn_readfirstlane(nfar - b0 < kWave ? nfar - b0 : kWave);
+      float* base = accw - 2;
+      const float m0 = lane == 0 ? 1.f : 0.f, m1 = lane == 1 ? 1.f : 0.f;
       for (int i = 0; i < n; i += kFarGroup) {
         FarLoads q[kFarGroup];
         int S[kFarGroup];
-        float ne[kFarGroup], f[kFarGroup];
+        float so[kFarGroup], sf[kFarGroup], fD[kFarGroup], omfD[kFarGroup];
 #pragma unroll
         for (int g = 0; g < kFarGroup; ++g) {
           const int j = i + g;  // < 64: i < n <= 64 and i is a multiple of kFarGroup
           S[g] = __builtin_amdgcn_readlane(rS, j);
-          ne[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rne), j));
-          f[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rf), j));
-          const float D = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rD), j));
-          far_load(tnp, __builtin_amdgcn_readlane(roff, j), D, q[g]);
+          so[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rso), j));
+          sf[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rsf), j));
+          fD[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rfD), j));
+          omfD[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(romfD), j));
+          far_load(tnp, __builtin_amdgcn_readlane(roff, j), 0.f, q[g]);
         }
         // unconditional stores: a branch on the charge would let the compiler sink a pair's loads
         // into it and wait on them there instead of overlapping them with the other pairs'
 #pragma unroll
         for (int g = 0; g < kFarGroup; ++g)
-          store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, ne[g], 1.f - f[g], f[g], q[g].D, S[g] >> 1, d.L, d.nout,
-                       (S[g] & 1) != 0);
+          store_stripe(base, lane, m0, m1, q[g].t0, q[g].t1, q[g].t2, so[g], sf[g], fD[g], omfD[g], S[g] >> 1, d.L,
+                       d.nout, (S[g] & 1) != 0);
       }
       for (uint64_t m = __ballot(bins); m != 0; m &= m - 1) {
         const int j = __builtin_ctzll(m);
