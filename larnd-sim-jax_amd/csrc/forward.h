@@ -530,6 +530,17 @@ __device__ __forceinline__ void far_load(const char* __restrict__ tnp, int off, 
   q.t2 = t[2 * kChunk];
 }
 
+// The same through a buffer descriptor of the table: the pair's byte offset goes in the scalar
+// offset field and the lane's in the vector one, so no per-pair address arithmetic.
+// 0x00020000: word 3 of a raw (stride 0) buffer descriptor on gfx9-family GPUs.
+constexpr int kBufDesc3 = 0x00020000;
+__device__ __forceinline__ void far_load_buf(__amdgpu_buffer_rsrc_t r, int lane4, int off, FarLoads& q) {
+  q.D = 0.f;
+  q.t0 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, lane4, off, 0));
+  q.t1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, lane4 + 4 * kChunk, off, 0));
+  q.t2 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, lane4 + 8 * kChunk, off, 0));
+}
+
 // Distinct LUT cells of a segment's diffusion bins that land on pixel offset a along one axis, with
 // the bins' weights summed per cell: bins on the same cell read the same template window and the
 // bin weights factorise (n_bin = n_e wx wy), so each distinct (cx, cy) is visited once with weight
@@ -831,7 +842,8 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   {
     const int nfar = c_cum[nsc];
     const int W4 = kWinPad * 4, nsb = d.nsb, side = 2 * d.nn + 1;
-    const char* tnp = reinterpret_cast<const char*>(tb.TNP + lane);
+    const __amdgpu_buffer_rsrc_t tnp =
+        __builtin_amdgcn_make_buffer_rsrc((void*)tb.TNP, 0, d.lx * d.ly * kWinPad * 4, kBufDesc3);
     for (int b0 = wid * kWave; b0 < nfar; b0 += WAVES * kWave) {
       const int p = b0 + lane;
       // padding lanes: an interior window at S = 2 with zero charge and correction (adds +0)
@@ -885,7 +897,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
           sf[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rsf), j));
           fD[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rfD), j));
           omfD[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(romfD), j));
-          far_load(tnp, __builtin_amdgcn_readlane(roff, j), 0.f, q[g]);
+          far_load_buf(tnp, lane * 4, __builtin_amdgcn_readlane(roff, j), q[g]);
         }
         // unconditional stores: a branch on the charge would let the compiler sink a pair's loads
         // into it and wait on them there instead of overlapping them with the other pairs'
