@@ -46,11 +46,6 @@ struct __align__(16) SegRec {
   float wx[5], wy[5];    // transverse weights along x and y
 };
 
-// The first 16 bytes of a SegRec: what the neighbour-stripe path reads.
-struct __align__(16) FarRec {
-  int k0, uxy;
-  float ne, f;
-};
 static_assert(sizeof(SegRec) == 80, "SegRec layout");
 
 // Self-trigger decisions of one pixel row, kept for the adjoint.  Point j < H is the

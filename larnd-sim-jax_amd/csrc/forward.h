@@ -512,27 +512,18 @@ __device__ __forceinline__ void store_stripe(float* base, int lane, float m0, fl
   }
 }
 
-// Neighbour stripe only (|a| or |b| > 1): raw response window, undiffused charge.  L <= 187.
-// The loads (`far_load`) are issued for a group of pairs before any of their stores, so a wave
-// keeps several independent pairs in flight.
+// Window values of one pair (and, for the centre pairs, the pair's correction): the loads of a
+// group of pairs are issued before any of their stores, so a wave keeps several independent pairs
+// in flight.
 struct FarLoads {
   float t0, t1, t2;
   float D;  // boundary correction Delta of the pair (charge included)
 };
 
-// Window loads of a neighbour-only pair: tnp = padded raw windows + this lane's byte offset;
-// off = the pair's row offset (bytes).
-__device__ __forceinline__ void far_load(const char* __restrict__ tnp, int off, float delta, FarLoads& q) {
-  const float* t = reinterpret_cast<const float*>(tnp + off);
-  q.D = delta;
-  q.t0 = t[0];
-  q.t1 = t[kChunk];
-  q.t2 = t[2 * kChunk];
-}
-
-// The same through a buffer descriptor of the table: the pair's byte offset goes in the scalar
-// offset field and the lane's in the vector one, so no per-pair address arithmetic.
-// 0x00020000: word 3 of a raw (stride 0) buffer descriptor on gfx9-family GPUs.
+// Neighbour-stripe window loads through a buffer descriptor of the padded raw-window table: the
+// pair's byte offset goes in the scalar offset field and the lane's in the vector one, so there
+// is no per-pair address arithmetic.  0x00020000: word 3 of a raw (stride 0) buffer descriptor on
+// gfx9-family GPUs.
 constexpr int kBufDesc3 = 0x00020000;
 __device__ __forceinline__ void far_load_buf(__amdgpu_buffer_rsrc_t r, int lane4, int off, FarLoads& q) {
   q.D = 0.f;
