@@ -561,7 +561,7 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
     if (pair_dead(S, G)) continue;
     pair_ctx(d, S, R.ne, R.f, G, pc);
     if (part == NX) {
-      if (a == 0 && b == 0) continue;
+      if ((a == 0 && b == 0) || a < -d.nn || a > d.nn || b < -d.nn || b > d.nn) continue;  // no stripe there
       stripe_adjoint(d, tb, R.uxy & 0xffff, R.uxy >> 16, pc, a, b, G, gN, gF);
     } else if (!xbin_adjoint<NB>(d, tb, R, adj + s, pc, a, b, kAdjBinItems ? part / NB : part,
                                  kAdjBinItems ? part % NB : -1, G, gN, gF, gS, gL)) {

@@ -20,6 +20,7 @@ struct Dev {
   // geometry
   int ntpc, nx, ny;
   int nsb, nb, nbsym, nn;  // sub-bins per pixel, diffusion bins, neighbour half width
+  int ne;                 // half width of the cell grid walked per row: max(nn, 1) (diffusion bins reach one pixel)
   int halo, gx, gy;        // cell grid = pixel grid + halo on each side
   int n_events;
   int64_t n_cells;

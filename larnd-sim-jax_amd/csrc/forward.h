@@ -101,14 +101,14 @@ __device__ __forceinline__ void seg_corrections(const Dev& d, const float2* __re
                                                 const SegRec& R, int ux, int uy, int64_t i, int64_t n,
                                                 float* __restrict__ dtot) {
   constexpr int SYM = (NB - 1) / 2;
-  const int W = 2 * d.nn + 1, nsb = d.nsb, cm = d.cm;
+  const int W = 2 * d.ne + 1, nsb = d.nsb, cm = d.cm;
   const float f = R.f;
   const float2* DNk = DN + (int64_t)R.k0 * d.lx * d.ly;
   const float2* DMk = DM + ((int64_t)R.k0 * d.nm + R.idx) * cm * cm;
   for (int ab = 0; ab < W * W; ++ab) {
-    const int a = ab / W - d.nn, b = ab % W - d.nn;
+    const int a = ab / W - d.ne, b = ab % W - d.ne;
     float delta = 0.f;
-    if (a != 0 || b != 0) {
+    if ((a != 0 || b != 0) && a >= -d.nn && a <= d.nn && b >= -d.nn && b <= d.nn) {
       const float2 D = DNk[offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb)];
       delta = R.ne * (D.x - f * D.y);
     }
@@ -700,7 +700,7 @@ template <int NT, class Cls, class Cnt>
 __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict__ rng, int cell, Cls cls, Cnt cnt,
                                            int* scratch, int* c_beg, int* c_cum, int* c_ab, int* ncls) {
   const int t = threadIdx.x;
-  const int side = 2 * d.nn + 1, ncell = side * side;
+  const int side = 2 * d.ne + 1, ncell = side * side;
   const int px = cell % d.gx - d.halo;
   const int py = (cell / d.gx) % d.gy - d.halo;
   const int evp = cell / (d.gx * d.gy);
@@ -712,7 +712,7 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
     ncls[0] = ncls[1] = ncls[2] = 0;
   }
   for (int i = t; i < ncell; i += NT) {
-    const int a = i / side - d.nn, b = i % side - d.nn;
+    const int a = i / side - d.ne, b = i % side - d.ne;
     const int c = cls(a, b);
     const int qx = px - a, qy = py - b;
     int key = -1, n = 0, beg = 0;
@@ -750,7 +750,7 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
         ++after;
       }
     }
-    const int a = i / side - d.nn, b = i % side - d.nn;
+    const int a = i / side - d.ne, b = i % side - d.ne;
     c_beg[rank] = s_beg[i];
     c_cum[rank] = before;
     c_ab[rank] = ((a + 64) << 8) | (b + 64);
@@ -832,7 +832,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   // row (segments near the pixel edge only) follow per block, picked out by a ballot.
   {
     const int nfar = c_cum[nsc];
-    const int W4 = kWinPad * 4, nsb = d.nsb, side = 2 * d.nn + 1;
+    const int W4 = kWinPad * 4, nsb = d.nsb, side = 2 * d.ne + 1;
     const __amdgpu_buffer_rsrc_t tnp =
         __builtin_amdgcn_make_buffer_rsrc((void*)tb.TNP, 0, d.lx * d.ly * kWinPad * 4, kBufDesc3);
     for (int b0 = wid * kWave; b0 < nfar; b0 += WAVES * kWave) {
@@ -860,12 +860,14 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         else if (b < 0) { kc += (nsb * -b - nsb / 2) * W4; mb = 1; }
         else zb = 1;
         const PairRec P = prs[sg];
-        const float D = dtot[(int64_t)((a + d.nn) * side + (b + d.nn)) * nseg + sg];
+        const float D = dtot[(int64_t)((a + d.ne) * side + (b + d.ne)) * nseg + sg];
+        // with no neighbour stripes (nn = 0) a side pixel gets its diffusion bins only
+        const float st = a >= -d.nn && a <= d.nn && b >= -d.nn && b <= d.nn ? 1.f : 0.f;
         roff = kc + ma * P.U + za * P.U0 + mb * P.V + zb * P.V0;
         rS = (P.S << 1) | (P.flags & 1);
         const float omf = 1.f - P.f;
-        rso = P.ne * omf;
-        rsf = P.ne * P.f;
+        rso = st * (P.ne * omf);
+        rsf = st * (P.ne * P.f);
         rfD = P.f * D;
         romfD = omf * D;
         rsg = sg;
@@ -909,7 +911,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     }
   }
   // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
-  const float* dtc = dtot + (int64_t)(d.nn * (2 * d.nn + 1) + d.nn) * nseg;
+  const float* dtc = dtot + (int64_t)(d.ne * (2 * d.ne + 1) + d.ne) * nseg;
   for (int c = nsc; c < nct; ++c) {
     const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
     const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);

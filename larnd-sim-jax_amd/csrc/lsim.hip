@@ -75,6 +75,7 @@ int make_dev(const lsim_ctx* c, const lsim_params* p, int n_events, Dev& d) {
   d.nb = k.nb_tran_diff_bins;
   d.nbsym = (d.nb - 1) / 2;
   d.nn = k.number_pix_neighbors;
+  d.ne = d.nn > 0 ? d.nn : 1;
   d.halo = c->halo;
   d.gx = d.nx + 2 * d.halo;
   d.gy = d.ny + 2 * d.halo;
@@ -366,8 +367,8 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
   hipStream_t st = (hipStream_t)stream;
   const lsim_config& k = c->cfg;
   if (nt < k.signal_length + 1 || nt > kMaxOut) return fail(LSIM_E_ARG, "LUT time axis out of range");
-  const int nsb = k.nb_sampling_bins_per_pixel, n = k.number_pix_neighbors;
-  const int maxc = nsb * n + nsb / 2 - 1;  // largest |offset| index a neighbour reads
+  const int nsb = k.nb_sampling_bins_per_pixel, n = k.number_pix_neighbors > 0 ? k.number_pix_neighbors : 1;
+  const int maxc = nsb * n + nsb / 2 - 1;  // largest |offset| index a neighbour (or a diffusion bin one pixel over) reads
   if (nx <= maxc || ny <= maxc || nx < c->cm || ny < c->cm)
     return fail(LSIM_E_ARG, "LUT too small for number_pix_neighbors (needs > " + std::to_string(maxc) + " bins per axis)");
   if (c->has_lut && (nx != c->lx || ny != c->ly || nt != c->nt)) free_tables(c);
@@ -403,7 +404,7 @@ int lsim_forward_query(lsim_ctx* c, int64_t n_seg, int32_t n_events, size_t* ws_
   int rc = make_dev(c, nullptr, n_events, d);
   if (rc) return rc;
   Ws w = layout(nullptr, n_seg, d.n_cells);
-  layout_center(w, nullptr, n_seg, d.nn);
+  layout_center(w, nullptr, n_seg, d.ne);
   *ws_bytes = w.bytes;
   if (max_rows_bound) {
     const int64_t plane_pixels = (int64_t)n_events * d.ntpc * d.nx * d.ny;
@@ -424,7 +425,7 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
   if (rc) return rc;
   const int64_t n = seg->n;
   Ws w = layout(ws, n, d.n_cells);
-  layout_center(w, ws, n, d.nn);
+  layout_center(w, ws, n, d.ne);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small: need " + std::to_string(w.bytes) + " bytes");
   SegIn in = make_segin(seg);
   int* status = status_dev;
@@ -533,7 +534,7 @@ int lsim_forward_rows(lsim_ctx* c, const lsim_params* p, int64_t n_seg, int32_t 
   rc = make_dev(c, p, n_events, d);
   if (rc) return rc;
   Ws w = layout(ws, n_seg, d.n_cells);
-  layout_center(w, ws, n_seg, d.nn);
+  layout_center(w, ws, n_seg, d.ne);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
   // rows never exceed segments; the per-row workspace buffers hold max(n, 1) entries
   if (max_rows > (n_seg > 1 ? n_seg : 1)) max_rows = n_seg > 1 ? n_seg : 1;

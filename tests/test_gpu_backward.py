@@ -58,6 +58,32 @@ def test_backward_matches_autograd(sim, params4, c1, frozen, seed):
     assert err.max() < GRAD_TOL
 
 
+def test_backward_without_neighbour_stripes(params4, lut, c1):
+    """number_pix_neighbors = 0: rows get no neighbour stripes but still the diffusion bins that
+    spill over from segments of the adjacent pixels (forward and adjoint walk the 3x3 cells)."""
+    from larndsim_amd.sim import Simulator
+    from oracle import ad_oracle as A
+    from oracle import oracle as O
+
+    p0 = params4.replace(number_pix_neighbors=0)
+    tr, f = c1
+    tr, f = tr[:400], f
+    fz = A.Frozen(p0, lut, tr, f, tables=O.build_templates(p0, lut))
+    s0 = Simulator(p0, lut)
+    print("[gpu] backward c1[:400] number_pix_neighbors 0", flush=True)
+    st = s0.forward(p0, tr, f, want_tape=True)
+    P = st.max_rows
+    assert np.array_equal(st.out["ticks"][:P].cpu().numpy(), fz.ticks[fz.valid_rows].astype(np.int64))
+    w_adc, w_z = _cotangents(fz, 3)
+    g_ref, _ = fz.vjp(w_adc, w_z)
+    g = s0.backward(st, torch.from_numpy(w_adc.astype(np.float32)), torch.from_numpy(w_z.astype(np.float32))).cpu().numpy()
+    print("gpu ", g)
+    print("ref ", g_ref)
+    mask = np.abs(g_ref) > 0
+    assert mask.sum() >= 5
+    assert _rel(g[mask], g_ref[mask]).max() < GRAD_TOL
+
+
 def test_autograd_function(params4, lut, c1, frozen):
     """Gradient through HipLUTSimulation.predict (the reference's LUTSimulation.predict path)."""
     from larndsim_amd.consts import GRAD_PARAMS

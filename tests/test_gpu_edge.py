@@ -65,7 +65,7 @@ def test_recombination_models(sim, params4, lut, tables, mode):
     compare_forward(sim, p, lut, tr, f, tables=tables, n_events=1, tag=mode)
 
 
-@pytest.mark.parametrize("nn,nb", [(2, 5), (4, 3), (1, 1)])
+@pytest.mark.parametrize("nn,nb", [(0, 5), (2, 5), (4, 3), (1, 1)])
 def test_other_configurations(params4, lut, nn, nb):
     from larndsim_amd.sim import Simulator
     from oracle import oracle as O
