@@ -74,3 +74,17 @@ def test_other_configurations(params4, lut, nn, nb):
     s = Simulator(p, lut)
     tr, f = _mixed(p, n_c1=300)
     compare_forward(s, p, lut, tr, f, tables=O.build_templates(p, lut), n_events=1, tag=f"n{nn}b{nb}")
+
+
+@pytest.mark.parametrize("kw", [dict(signal_length=187), dict(signal_length=100), dict(MAX_ADC_VALUES=4)],
+                         ids=["L187", "L100", "H4"])
+def test_window_length_and_hit_slots(params4, lut, kw):
+    """The longest window the kernels take (187 samples: three 63-lane chunks), a shorter one, and
+    fewer self-trigger passes than the compiled maximum."""
+    from larndsim_amd.sim import Simulator
+    from oracle import oracle as O
+
+    p = params4.replace(**kw)
+    s = Simulator(p, lut)
+    tr, f = _mixed(p, n_c1=300)
+    compare_forward(s, p, lut, tr, f, tables=O.build_templates(p, lut), n_events=1, tag=str(kw))
