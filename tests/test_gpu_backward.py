@@ -58,19 +58,22 @@ def test_backward_matches_autograd(sim, params4, c1, frozen, seed):
     assert err.max() < GRAD_TOL
 
 
-def test_backward_without_neighbour_stripes(params4, lut, c1):
-    """number_pix_neighbors = 0: rows get no neighbour stripes but still the diffusion bins that
-    spill over from segments of the adjacent pixels (forward and adjoint walk the 3x3 cells)."""
+@pytest.mark.parametrize("kw", [dict(number_pix_neighbors=0), dict(MAX_ADC_VALUES=4), dict(signal_length=187)],
+                         ids=["n0", "H4", "L187"])
+def test_backward_other_settings(params4, lut, c1, kw):
+    """number_pix_neighbors = 0 (rows get no neighbour stripes but still the diffusion bins that
+    spill over from segments of the adjacent pixels: forward and adjoint walk the 3x3 cells), fewer
+    self-trigger passes than the compiled maximum, and the longest window."""
     from larndsim_amd.sim import Simulator
     from oracle import ad_oracle as A
     from oracle import oracle as O
 
-    p0 = params4.replace(number_pix_neighbors=0)
+    p0 = params4.replace(**kw)
     tr, f = c1
     tr, f = tr[:400], f
     fz = A.Frozen(p0, lut, tr, f, tables=O.build_templates(p0, lut))
     s0 = Simulator(p0, lut)
-    print("[gpu] backward c1[:400] number_pix_neighbors 0", flush=True)
+    print(f"[gpu] backward c1[:400] {kw}", flush=True)
     st = s0.forward(p0, tr, f, want_tape=True)
     P = st.max_rows
     assert np.array_equal(st.out["ticks"][:P].cpu().numpy(), fz.ticks[fz.valid_rows].astype(np.int64))
