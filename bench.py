@@ -40,6 +40,45 @@ def algorithmic_bytes_fwd(n_seg: int, rows: int, H: int = 10, nout: int = 2000) 
     return 24 * n_seg + LUT_BYTES + rows * nout * 4 + rows * (8 + H * 8)
 
 
+def algorithmic_bytes_fwdbwd(n_seg: int, rows: int, H: int = 10, nout: int = 2000) -> int:
+    """Compulsory HBM bytes of a forward+backward: the forward's, the inputs and the raw LUT read
+    again by the backward, d loss / d wfs written and read once, and the 7 partial gradients of each
+    row workgroup (two adjoint passes)."""
+    return algorithmic_bytes_fwd(n_seg, rows, H, nout) + 24 * n_seg + LUT_BYTES + 2 * rows * nout * 4 + 2 * rows * 56
+
+
+# On-chip work of the algorithm per segment (n neighbours, nb x nb diffusion bins, L-sample windows):
+# scatter-adds into pixel x tick rows and LUT gathers, counted as LDS bytes (8 per add, 4 per gather).
+def onchip_per_segment(n: int = 4, nb: int = 5, L: int = 150) -> dict:
+    B, K = nb * nb, (2 * n + 1) ** 2
+    scatter = (B + K) * (2 * L + 2)
+    gathers = 3 * B * L + K * L + 3 * B + 3 * K
+    return dict(scatter_adds=scatter, lut_gathers=gathers, lds_bytes=8 * scatter + 4 * gathers)
+
+
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
+
+
+def copy_gbs(dev, nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """Achievable HBM bandwidth: a plain device copy of nbytes (read + write counted)."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    del src, dst
+    return 2 * nbytes / t / 1e9
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -108,8 +147,10 @@ def batched_fwd_bwd(simr, params, dev, steps, fd_baseline=True):
     torch.cuda.synchronize()
     dtf = (time.perf_counter() - t0) / fsteps
     del g
+    bb = algorithmic_bytes_fwdbwd(NF, frun.rows)
     res = {"batched_fwd_bwd": dict(events=32, segments=NF, rows=frun.rows, ms_per_step=dtf * 1e3, segments_per_s=NF / dtf,
-                                   graph=True, wfs_stored=False)}
+                                   graph=True, wfs_stored=False, algorithmic_bytes=bb,
+                                   hbm_frac=bb / dtf / 1e9 / HBM_PEAK_GBS)}
     del frun
     if not fd_baseline:
         log(f"[bench] batched fwd+bwd {NF} segments {dtf * 1e3:.2f} ms -> {NF / dtf:.3e} seg/s")
@@ -240,7 +281,9 @@ def main():
         torch.cuda.synchronize()
         dtfd = (time.perf_counter() - t0) / max(1, a.steps // 5)
         run.set_params(params)
-        extra["fwd_bwd"] = dict(ms_per_step=dtb * 1e3, segments_per_s=N / dtb)
+        bfb = algorithmic_bytes_fwdbwd(N, P)
+        extra["fwd_bwd"] = dict(ms_per_step=dtb * 1e3, segments_per_s=N / dtb, algorithmic_bytes=bfb,
+                                hbm_frac=bfb / dtb / 1e9 / HBM_PEAK_GBS)
         extra["fd_gradient_baseline"] = dict(forwards=len(fd_params), ms_per_gradient=dtfd * 1e3,
                                              segments_per_s=N / dtfd, adjoint_speedup=dtfd / dtb)
         log(f"[bench] fwd+bwd {dtb * 1e3:.3f} ms; FD gradient {dtfd * 1e3:.3f} ms (x{dtfd / dtb:.1f})")
@@ -274,8 +317,10 @@ def main():
             g.replay()
         torch.cuda.synchronize()
         dtg = (time.perf_counter() - t0) / bsteps
+        bbt = algorithmic_bytes_fwd(NB, brun.rows)
         extra["batched"] = dict(events=128, segments=NB, rows=brun.rows, ms_per_step=dtg * 1e3,
-                                segments_per_s=NB / dtg, graph=True)
+                                segments_per_s=NB / dtg, graph=True, algorithmic_bytes=bbt,
+                                hbm_frac=bbt / dtg / 1e9 / HBM_PEAK_GBS)
         log(f"[bench] batched {NB} segments {dtg * 1e3:.2f} ms -> {NB / dtg:.3e} seg/s")
         del brun, btr, g
 
@@ -367,6 +412,13 @@ def main():
         cpu = cpu_baseline(params, lut, tracks, fields, a.cpu_sample)
         log(f"[bench] CPU {cpu['value']:.1f} seg/s")
 
+    achievable = copy_gbs(dev)
+    log(f"[bench] device copy {achievable:.0f} GB/s")
+    oc = onchip_per_segment(int(params.number_pix_neighbors), int(params.nb_tran_diff_bins), int(params.signal_length))
+    onchip = dict(per_segment=oc, lds_bytes=oc["lds_bytes"] * N, lds_gbs=oc["lds_bytes"] * N / (k_avg * 1e-3) / 1e9,
+                  lds_peak_gbs=LDS_PEAK_GBS, lds_frac=oc["lds_bytes"] * N / (k_avg * 1e-3) / 1e9 / LDS_PEAK_GBS,
+                  note="the algorithm's scatter-adds and LUT gathers as LDS bytes over the waveform kernel's time")
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "rows_fwd_c2_traffic.json")
     if os.path.exists(tpath):
@@ -391,7 +443,9 @@ def main():
                    "segments": N, "pixel_rows": P, "row_waves": waves},
         "roofline": {"bound": "hbm", "kernel": "k_rows_fwd", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": bytes_fwd, "kernel_ms": k_avg},
+                     "algorithmic_bytes": bytes_fwd, "kernel_ms": k_avg, "achievable": achievable,
+                     "forward_frac": bytes_fwd / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "onchip": onchip,
         "cpu_baseline": cpu,
     }
     line.update(extra)
