@@ -191,6 +191,43 @@ int lsim_backward(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg,
 int lsim_rbf_sums(const float *x, int64_t nx, const int32_t *lo, const int32_t *hi, const float *y, const float *c,
                   int64_t ny, float sigma, double *out, void *stream);
 
+/* ---- Track chopping and batch padding (the data step before the path) ------------------------ */
+
+/* Column indices of the float32 (N, n_fields) track array (the reference's `fields.index`). */
+typedef struct {
+  int32_t n_fields;                     /* F: floats per row, 1 .. 64 */
+  int32_t start[3], end[3], mid[3];     /* x_start y_start z_start, x_end y_end z_end, x y z */
+  int32_t dE, dx;
+  int32_t event;                        /* eventID column, or -1 */
+  int32_t n_zero, zero[8];              /* set to 0 on invalidated rows (n_electrons, dE, dEdx, dx, long_diff, tran_diff) */
+  int32_t n_neg_one, neg_one[8];        /* set to -1 on invalidated rows (trackID, pixel_plane) */
+} lsim_chop_layout;
+
+/* Workspace bytes of lsim_chop_count / lsim_chop_fill for n_tracks track rows. */
+int lsim_chop_query(int64_t n_tracks, size_t *ws_bytes);
+
+/* Step counts of chop_tracks (optimize/dataio.py:63-106, `nsteps = max(ceil(length / precision), 1)`,
+ * float32 length) and their exclusive scan, kept in the workspace.  info [3] device int64:
+ * info[0] = total output rows, info[1] = tracks whose length is not finite or needs more than
+ * 2^31 - 1 steps (counted as one step; the reference fails on them), info[2] = 0. */
+int lsim_chop_count(const lsim_chop_layout *layout, const float *tracks, int64_t n_tracks, double precision,
+                    void *ws, size_t ws_bytes, int64_t *info, void *stream);
+
+/* chop_tracks (optimize/dataio.py:63-106) into out [n_rows][F] float32, bit-identical to the
+ * reference under NumPy 2 rounding, followed by the batch padding of TracksDataset.pad_batch
+ * (:357-373): rows n_steps .. n_rows - 1 are zero rows invalidated as _invalidate_rows (:347-355)
+ * does (eventID -1, zero columns 0, -1 columns -1).  n_steps must be info[0] of the matching
+ * lsim_chop_count call and n_rows >= n_steps.
+ * Optional event remap (remap_event_ids_to_local, :47-61): when src_events [n_tracks] is not NULL,
+ * each track's eventID becomes the index of src_events[i] in the sorted batch_events
+ * [n_batch_events]; ids not found are written as -1 and counted in info[2] (the reference raises).
+ * When n_local_events >= 0, rows whose eventID e >= 0 has trunc(e) >= n_local_events are
+ * invalidated (pad_batch with a batch index); -1 disables it. */
+int lsim_chop_fill(const lsim_chop_layout *layout, const float *tracks, int64_t n_tracks, double precision,
+                   const void *ws, size_t ws_bytes, int64_t n_steps, int64_t n_rows, const int64_t *src_events,
+                   const int64_t *batch_events, int64_t n_batch_events, int64_t n_local_events, float *out,
+                   int64_t *info, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

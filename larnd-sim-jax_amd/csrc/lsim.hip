@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "backward.h"
+#include "chop.h"
 #include "loss.h"
 
 using namespace lsim;
@@ -700,6 +701,97 @@ int lsim_rbf_sums(const float* x, int64_t nx, const int32_t* lo, const int32_t* 
   const float inv = 1.f / (2.f * sigma * sigma);
   hipLaunchKernelGGL(k_rbf_sums, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                      (const float4*)x, nx, lo, hi, (const float4*)y, (const float4*)c, inv, out);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+// ---- track chopping ----------------------------------------------------------------------------
+
+static int chop_layout(const lsim_chop_layout* l, ChopLayout& L) {
+  if (!l) return fail(LSIM_E_ARG, "NULL layout");
+  L.F = l->n_fields;
+  if (L.F < 1 || L.F > kChopMaxFields) return fail(LSIM_E_ARG, "n_fields must be in 1 .. 64");
+  auto col = [&](int32_t c) { return c >= 0 && c < L.F; };
+  for (int a = 0; a < 3; ++a) {
+    L.s[a] = l->start[a];
+    L.e[a] = l->end[a];
+    L.m[a] = l->mid[a];
+    if (!col(L.s[a]) || !col(L.e[a]) || !col(L.m[a])) return fail(LSIM_E_ARG, "position column out of range");
+  }
+  L.dE = l->dE;
+  L.dx = l->dx;
+  L.ev = l->event;
+  if (!col(L.dE) || !col(L.dx)) return fail(LSIM_E_ARG, "dE / dx column out of range");
+  if (L.ev != -1 && !col(L.ev)) return fail(LSIM_E_ARG, "event column out of range");
+  if (l->n_zero < 0 || l->n_zero > kChopMaxCols || l->n_neg_one < 0 || l->n_neg_one > kChopMaxCols)
+    return fail(LSIM_E_ARG, "at most 8 zero and 8 -1 columns");
+  L.nzero = l->n_zero;
+  L.nneg = l->n_neg_one;
+  for (int c = 0; c < L.nzero; ++c)
+    if (!col(L.zero[c] = l->zero[c])) return fail(LSIM_E_ARG, "zero column out of range");
+  for (int c = 0; c < L.nneg; ++c)
+    if (!col(L.neg[c] = l->neg_one[c])) return fail(LSIM_E_ARG, "-1 column out of range");
+  return LSIM_OK;
+}
+
+static size_t chop_ws_bytes(int64_t n) { return (size_t)(n + scan_tiles(n) + 4) * sizeof(int64_t); }
+
+static int chop_check(const float* tracks, int64_t n, double precision, size_t ws_bytes) {
+  if (n < 0) return fail(LSIM_E_ARG, "n_tracks must be >= 0");
+  if (n >= ((int64_t)1 << 40)) return fail(LSIM_E_ARG, "too many tracks");
+  if (!(precision > 0) || !std::isfinite(precision)) return fail(LSIM_E_ARG, "precision must be finite and > 0");
+  if (n > 0 && !tracks) return fail(LSIM_E_ARG, "NULL tracks");
+  if (ws_bytes < chop_ws_bytes(n)) return fail(LSIM_E_SPACE, "chop workspace too small");
+  return LSIM_OK;
+}
+
+int lsim_chop_query(int64_t n_tracks, size_t* ws_bytes) {
+  if (n_tracks < 0 || !ws_bytes) return fail(LSIM_E_ARG, "bad arguments");
+  *ws_bytes = chop_ws_bytes(n_tracks);
+  return LSIM_OK;
+}
+
+int lsim_chop_count(const lsim_chop_layout* layout, const float* tracks, int64_t n_tracks, double precision, void* ws,
+                    size_t ws_bytes, int64_t* info, void* stream) {
+  ChopLayout L;
+  int rc = chop_layout(layout, L);
+  if (rc != LSIM_OK) return rc;
+  if ((rc = chop_check(tracks, n_tracks, precision, ws_bytes)) != LSIM_OK) return rc;
+  if (!ws || !info) return fail(LSIM_E_ARG, "NULL workspace or info");
+  hipStream_t st = (hipStream_t)stream;
+  int64_t* off = (int64_t*)ws;
+  int64_t* tmp = off + n_tracks;
+  hipLaunchKernelGGL((k_fill<int64_t>), dim3(1), dim3(64), 0, st, info, (int64_t)3, (int64_t)0);
+  HIP_TRY(hipGetLastError());
+  if (n_tracks == 0) return LSIM_OK;
+  hipLaunchKernelGGL(k_chop_count, dim3((unsigned)((n_tracks + 255) / 256)), dim3(256), 0, st, L, tracks, n_tracks,
+                     (float)precision, off, info);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY((exclusive_scan<int64_t, int64_t>(off, off, n_tracks, tmp, info, st)));
+  return LSIM_OK;
+}
+
+int lsim_chop_fill(const lsim_chop_layout* layout, const float* tracks, int64_t n_tracks, double precision,
+                   const void* ws, size_t ws_bytes, int64_t n_steps, int64_t n_rows, const int64_t* src_events,
+                   const int64_t* batch_events, int64_t n_batch_events, int64_t n_local_events, float* out,
+                   int64_t* info, void* stream) {
+  ChopLayout L;
+  int rc = chop_layout(layout, L);
+  if (rc != LSIM_OK) return rc;
+  if ((rc = chop_check(tracks, n_tracks, precision, ws_bytes)) != LSIM_OK) return rc;
+  if (n_steps < 0 || n_rows < n_steps) return fail(LSIM_E_ARG, "need 0 <= n_steps <= n_rows");
+  if (n_steps > 0 && n_tracks == 0) return fail(LSIM_E_ARG, "n_steps > 0 with no tracks");
+  if (n_steps < n_tracks) return fail(LSIM_E_ARG, "n_steps below the track count (every track has a step)");
+  if (n_rows >= ((int64_t)1 << 40)) return fail(LSIM_E_ARG, "too many rows");
+  if (src_events && (L.ev < 0 || n_batch_events < 0 || (n_batch_events > 0 && !batch_events)))
+    return fail(LSIM_E_ARG, "event remap needs the event column and the batch event ids");
+  if (n_local_events < -1) return fail(LSIM_E_ARG, "n_local_events must be >= -1");
+  if (n_rows == 0) return LSIM_OK;
+  if (!out || !info || (n_steps > 0 && !ws)) return fail(LSIM_E_ARG, "NULL output, info or workspace");
+  const size_t lds = (size_t)kChopRows * L.F * sizeof(float);
+  hipLaunchKernelGGL(k_chop_fill, dim3((unsigned)((n_rows + kChopRows - 1) / kChopRows)), dim3(kChopRows), lds,
+                     (hipStream_t)stream, L, tracks, n_tracks, precision, (const int64_t*)ws, n_steps, n_rows,
+                     src_events, batch_events, n_batch_events, n_local_events, out, info);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }

@@ -91,6 +91,36 @@ def make_batch(rng, sizes, **kw) -> np.ndarray:
     return np.concatenate([make_event(rng, int(n), event_id=e, **kw) for e, n in enumerate(sizes)], axis=0)
 
 
+def make_tracks(rng, n_tracks: int, length_range=(0.2, 1.8), n_events: int = 1) -> np.ndarray:
+    """Unchopped track rows (one straight piece each, as edep-sim writes them) for the chop step.
+
+    Starts uniform in the drift cube, isotropic directions, lengths U(length_range) cm, dE/dx
+    Moyal around 2.1 MeV/cm, dE = dEdx * length, dx = length, midpoint x/y/z; rows sorted by
+    eventID 0 .. n_events - 1, trackID the row index within the event.
+    """
+    out = np.zeros((n_tracks, len(SEGMENT_FIELDS)), dtype=np.float32)
+    col = {f: i for i, f in enumerate(SEGMENT_FIELDS)}
+    start = rng.uniform(CUBE_LO, CUBE_HI, size=(n_tracks, 3))
+    u = rng.normal(size=(n_tracks, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    length = rng.uniform(length_range[0], length_range[1], n_tracks)
+    end = start + u * length[:, None]
+    ev = np.sort(rng.integers(0, n_events, n_tracks))
+    first = np.searchsorted(ev, ev)
+    dedx = _landau_dedx(rng, n_tracks)
+    for k, c in enumerate("xyz"):
+        out[:, col[c + "_start"]] = start[:, k]
+        out[:, col[c + "_end"]] = end[:, k]
+        out[:, col[c]] = 0.5 * (start[:, k] + end[:, k])
+    out[:, col["eventID"]] = ev
+    out[:, col["trackID"]] = np.arange(n_tracks) - first
+    out[:, col["pdgId"]] = 13
+    out[:, col["dx"]] = length
+    out[:, col["dEdx"]] = dedx
+    out[:, col["dE"]] = (dedx * length).astype(np.float32)
+    return out
+
+
 # --- the five workloads -------------------------------------------------------------------
 
 def workload(name: str):

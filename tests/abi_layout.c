@@ -36,5 +36,10 @@ int main(void) {
     OFF(lsim_fwd_out, hit_x); OFF(lsim_fwd_out, hit_y); OFF(lsim_fwd_out, hit_z); OFF(lsim_fwd_out, hit_tick);
     OFF(lsim_fwd_out, hit_prob); OFF(lsim_fwd_out, hit_event); OFF(lsim_fwd_out, hit_pixel);
     OFF(lsim_fwd_out, hit_index); OFF(lsim_fwd_out, status);
+    SZ(lsim_chop_layout);
+    OFF(lsim_chop_layout, n_fields); OFF(lsim_chop_layout, start); OFF(lsim_chop_layout, end);
+    OFF(lsim_chop_layout, mid); OFF(lsim_chop_layout, dE); OFF(lsim_chop_layout, dx); OFF(lsim_chop_layout, event);
+    OFF(lsim_chop_layout, n_zero); OFF(lsim_chop_layout, zero); OFF(lsim_chop_layout, n_neg_one);
+    OFF(lsim_chop_layout, neg_one);
     return 0;
 }

@@ -68,7 +68,7 @@ def test_struct_layouts_match_header(tmp_path):
     lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     c = dict(ln.split() for ln in lines if ln.strip())
     types = {"lsim_geometry": _lib.Geometry, "lsim_config": _lib.Config, "lsim_params": _lib.ParamsC,
-             "lsim_segments": _lib.Segments, "lsim_fwd_out": _lib.FwdOut}
+             "lsim_segments": _lib.Segments, "lsim_fwd_out": _lib.FwdOut, "lsim_chop_layout": _lib.ChopLayout}
     n = 0
     for key, val in c.items():
         parts = key.split(".")
@@ -78,7 +78,7 @@ def test_struct_layouts_match_header(tmp_path):
         else:
             assert getattr(T, parts[2]).offset == int(val), key
         n += 1
-    assert n == 75
+    assert n == 87
 
 
 def C_sizeof(T):
