@@ -13,6 +13,7 @@ Adam fit of the 7 parameters on those 32 events (steps/s, parameter recovery), a
 baseline (the float32 NumPy restatement of the reference algorithm on a bounded sample, on the
 host cores).
 
+``--chop-only`` runs only the track-chopping leg (350k track rows -> ~3.5M segment rows).
 ``--batched-bwd-only`` runs only the 32-event forward+backward graph replays (for kernel profiles).
 ``--copy-calibrate`` only runs a plain 256 MiB device copy five times: profiled with the same
 ``rocprofv3 --pmc FETCH_SIZE`` / ``WRITE_SIZE`` passes it gives the counter-to-bytes factor
@@ -106,6 +107,82 @@ def cpu_baseline(params, lut, tracks, fields, n_sample: int):
                 host_cpu_count=os.cpu_count())
 
 
+def chop_leg(dev, steps, n_tracks=350_000, precision=0.1, cpu=True):
+    """Track chopping (the step before the path): 350k edep-like rows of 0.2-1.8 cm chopped at the
+    dataset's 0.1 cm into ~3.5M segment rows (the size of the 32-event batch).  The fill kernel is
+    timed with events on torch's current stream (where it is launched); the whole call (count, one
+    host read of the total, allocation, fill) by wall clock.  Bytes: the track rows read twice
+    (count and fill), the int64 offsets written and read, the segment rows written."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from larndsim_amd import _lib, synthetic
+    from larndsim_amd.dataio import _Chopper, chop_tracks
+
+    fields = tuple(synthetic.SEGMENT_FIELDS)
+    t = synthetic.make_tracks(np.random.default_rng(21), n_tracks, (0.2, 1.8), n_events=32)
+    td = torch.from_numpy(t).to(dev)
+    F = len(fields)
+    for _ in range(2):
+        out = chop_tracks(td, fields, precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = chop_tracks(td, fields, precision)
+    torch.cuda.synchronize()
+    call = (time.perf_counter() - t0) / steps
+    M = out.shape[0]
+    ch = _Chopper(td, fields, precision, dev)
+    st = ch._stream()
+    info = ch.info
+
+    def fill():
+        _lib.check(ch.lib.lsim_chop_fill(C.byref(ch.lay), C.c_void_p(ch.t.data_ptr()), ch.n, C.c_double(precision),
+                                         C.c_void_p(ch.ws.data_ptr()), ch.ws.numel(), M, M, None, None, 0, -1,
+                                         C.c_void_p(out.data_ptr()), C.c_void_p(info.data_ptr()), st))
+
+    def count():
+        _lib.check(ch.lib.lsim_chop_count(C.byref(ch.lay), C.c_void_p(ch.t.data_ptr()), ch.n, C.c_double(precision),
+                                          C.c_void_p(ch.ws.data_ptr()), ch.ws.numel(), C.c_void_p(info.data_ptr()),
+                                          st))
+
+    res = {}
+    for name, fn in (("fill", fill), ("count_scan", count)):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / steps
+    b_fill = n_tracks * F * 4 + n_tracks * 8 + M * F * 4
+    b_count = n_tracks * F * 4 + 2 * n_tracks * 8
+    leg = dict(workload=f"{n_tracks} track rows (0.2-1.8 cm) chopped at {precision} cm", tracks=n_tracks,
+               segments=M, ms_per_call=call * 1e3, segments_per_s=M / call, fill_kernel_ms=res["fill"],
+               count_scan_ms=res["count_scan"], fill_bytes=b_fill, fill_gbs=b_fill / (res["fill"] * 1e-3) / 1e9,
+               fill_hbm_frac=b_fill / (res["fill"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               kernels_gbs=(b_fill + b_count) / ((res["fill"] + res["count_scan"]) * 1e-3) / 1e9)
+    if cpu:
+        # the vectorised NumPy restatement of chop_tracks on the host (the reference loops in Python per track)
+        sys.path.insert(0, ROOT)
+        from threadpoolctl import threadpool_limits
+
+        from oracle import chop as OC
+
+        with threadpool_limits(limits=1):
+            t1 = time.perf_counter()
+            ref = OC.chop_tracks(t, fields, precision)
+            dtc = time.perf_counter() - t1
+        leg["cpu_baseline"] = dict(value=ref.shape[0] / dtc, unit="segments/s", cores=1, kind="port",
+                                   sample=f"all {n_tracks} rows, vectorised float32/float64 NumPy restatement, "
+                                          f"1 thread, {dtc:.2f} s")
+    del td, out, ch
+    return leg
+
+
 def batched_fwd_bwd(simr, params, dev, steps, fd_baseline=True):
     """32 uneven events (20k-200k segments): forward + backward replayed as one hipGraph, waveform
     rows recomputed (not stored), against the per-event forward-then-finite-difference gradient."""
@@ -190,6 +267,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--fit-steps", type=int, default=200, help="Adam steps of the 32-event gradient fit (0: skip)")
     ap.add_argument("--copy-calibrate", action="store_true")
+    ap.add_argument("--no-chop", action="store_true", help="skip the track-chopping leg")
+    ap.add_argument("--chop-only", action="store_true", help="profiling: only the track-chopping leg")
     ap.add_argument("--batched-bwd-only", action="store_true",
                     help="profiling: only the 32-event batched forward+backward graph replays (no baseline)")
     a = ap.parse_args()
@@ -215,6 +294,9 @@ def main():
     dev = torch.device("cuda")
     params = consts.default_params(number_pix_neighbors=4, RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0)
     lut = synthetic.make_lut(0)
+    if a.chop_only:
+        print(json.dumps(chop_leg(dev, a.steps, cpu=not a.no_cpu)))
+        return
     if a.batched_bwd_only:
         if a.waves:
             S.set_row_waves(a.waves)
@@ -327,6 +409,13 @@ def main():
     # ---- batched forward + backward: 32 uneven events, one graph replay, vs per-event FD ----
     if not a.no_batched_bwd:
         extra.update(batched_fwd_bwd(simr, params, dev, a.steps))
+
+    # ---- track chopping (the data step before the path) ----
+    if not a.no_chop:
+        log("[bench] track chopping")
+        extra["chop"] = chop_leg(dev, a.steps, cpu=not a.no_cpu)
+        log(f"[bench] chop {extra['chop']['segments']} rows: call {extra['chop']['ms_per_call']:.3f} ms, fill kernel "
+            f"{extra['chop']['fill_kernel_ms']:.3f} ms ({extra['chop']['fill_gbs']:.0f} GB/s)")
 
     # ---- gradient fit: 32 uneven events, Adam on the 7 parameters from nominal to a perturbed target ----
     if a.fit_steps > 0:

@@ -182,14 +182,26 @@ int lsim_backward(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg,
                   size_t ws_bytes, int64_t max_rows, const lsim_fwd_out *fwd, const float *d_adc, const float *d_z,
                   void *bwd_ws, size_t bwd_ws_bytes, double *d_params, void *stream);
 
-/* Fit loss: event-blocked Gaussian-kernel sums of the hit-cloud MMD in mse_adc
- * (src/larndsim/losses_jax.py:13-39 rbf_kernel / mmd, :58-80 mse_adc), without forming the
- * kernel matrices: out[i][k] = sum_{j = lo[i]}^{hi[i]-1} exp(-|x_i - y_j|^2 / (2 sigma^2)) c[j][k].
- * x [nx][4] and y [ny][4] float32 (x, y, z, unused), c [ny][4] float32 coefficients, lo/hi [nx]
+/* Fit losses: event-blocked Gaussian-kernel sums of the hit-cloud MMD in mse_adc
+ * (src/larndsim/losses_jax.py:13-39 rbf_kernel / mmd, :58-80 mse_adc) and of the intensity of
+ * nll_loss (:329-361), without forming the kernel matrices:
+ * out[i][k] = sum_{j = lo[i]}^{hi[i]-1} exp(-|x_i - y_j|^2 / (2 sigma^2)) c[j][k].
+ * x [nx][4] and y [ny][4] float32 4-D points (3-D callers set the fourth coordinate to 0), c [ny][4]
+ * float32 coefficients, lo/hi [nx]
  * int32 with 0 <= lo[i] <= hi[i] <= ny (the points of x_i's event in y, both clouds sorted by
  * event), out [nx][4] float64.  Distances and exponentials in float32, sums in float64. */
 int lsim_rbf_sums(const float *x, int64_t nx, const int32_t *lo, const int32_t *hi, const float *y, const float *c,
                   int64_t ny, float sigma, double *out, void *stream);
+
+/* Fit loss: event-blocked nearest neighbour of the chamfer distance (losses_jax.py:195-229
+ * chamfer_distance_3d_og, :231-249 chamfer_3d): for each x_i, the index j in [lo[i], hi[i]) of the
+ * y point at the smallest squared distance (dx^2 + dy^2) + dz^2 (float32; the first j on ties, as
+ * argmin) and that distance.  x [nx][4], y [ny][4] float32 (x, y, z, unused), lo/hi as for
+ * lsim_rbf_sums.  The first n_origin queries also consider the zero-weight padding point at the
+ * origin that the reference's pad_size padding adds after the real points: when it is strictly
+ * nearer, idx[i] = -1.  No candidate: idx[i] = -1, dmin[i] = +inf.  idx [nx] int32, dmin [nx] float32. */
+int lsim_nearest(const float *x, int64_t nx, const int32_t *lo, const int32_t *hi, const float *y, int64_t ny,
+                 int64_t n_origin, int32_t *idx, float *dmin, void *stream);
 
 /* ---- Track chopping and batch padding (the data step before the path) ------------------------ */
 

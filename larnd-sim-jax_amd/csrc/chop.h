@@ -4,9 +4,10 @@
 // Every input track row of the float32 (N, F) array becomes n = max(ceil(length / precision), 1)
 // rows of the output.  Two launches after the step counts' exclusive scan:
 //   k_chop_count: one thread per track -> n (int64), the scan gives each track its first row;
-//   k_chop_fill:  one thread per output row builds the row in LDS (track found by binary search of
-//                 the row index in the offsets), then the block stores its rows as one contiguous
-//                 run of floats, so HBM writes are fully coalesced.
+//   k_chop_fill:  one thread per output row builds the row in LDS (the block finds its first track
+//                 by a 256-ary search of the offsets, stages the next 256 offsets in LDS and each
+//                 thread searches those), then the block stores its rows as one contiguous run of
+//                 floats (float4 when aligned), so HBM writes are fully coalesced.
 // The rounding follows NumPy 2's rules on the reference's float32 array exactly:
 //   length, direction, the step count and the per-step dE in float32; step positions
 //   s + (k precision) d and s + (precision (k + 1)) d in float64; the last step's dE and dx in
@@ -15,6 +16,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <climits>
 
 namespace lsim {
 
@@ -81,6 +84,23 @@ __device__ __forceinline__ void chop_invalidate(const ChopLayout& L, float* row)
   for (int c = 0; c < L.nneg; ++c) row[L.neg[c]] = -1.f;
 }
 
+// Block-cooperative search for the last track whose first row is <= r0 (off[0] = 0 <= r0): each
+// round the 256 threads test 256 evenly spaced offsets and the barrier count of true tests picks
+// the bucket, so n tracks take ceil(log_256 n) dependent loads instead of log_2 n.
+__device__ __forceinline__ int64_t chop_block_track(const int64_t* __restrict__ off, int64_t n, int64_t r0) {
+  int64_t lo = 0, hi = n;  // answer in [lo, hi)
+  while (hi - lo > 1) {
+    const int64_t stride = (hi - lo + kChopRows - 1) / kChopRows;
+    const int64_t idx = lo + (int64_t)threadIdx.x * stride;
+    const int pred = idx < hi && off[idx] <= r0;
+    const int cnt = __syncthreads_count(pred);  // >= 1: off[lo] <= r0
+    lo += (int64_t)(cnt - 1) * stride;
+    hi = lo + stride < hi ? lo + stride : hi;
+  }
+  return lo;
+}
+
+template <bool V4>
 __global__ __launch_bounds__(kChopRows) void k_chop_fill(ChopLayout L, const float* __restrict__ tracks, int64_t n,
                                                         double prec, const int64_t* __restrict__ off, int64_t n_steps,
                                                         int64_t n_rows, const int64_t* __restrict__ src_ev,
@@ -88,26 +108,38 @@ __global__ __launch_bounds__(kChopRows) void k_chop_fill(ChopLayout L, const flo
                                                         int64_t n_local, float* __restrict__ out,
                                                         int64_t* __restrict__ info) {
 #pragma clang fp contract(off)
-  extern __shared__ float tile[];
+  extern __shared__ __align__(16) float tile[];
+  __shared__ int64_t s_off[kChopRows + 1];
   const int F = L.F;
   const int64_t r0 = (int64_t)blockIdx.x * kChopRows;
   const int64_t r = r0 + threadIdx.x;
   float* row = tile + threadIdx.x * F;
+  // The block's rows belong to tracks t0 .. t0 + m - 1 (m <= 256: every track has a row); their
+  // first rows go to LDS and each thread finds its track there.
+  int64_t t0 = 0;
+  int m = 0;
+  if (r0 < n_steps) {  // block-uniform
+    t0 = chop_block_track(off, n, r0);
+    const int64_t ti = t0 + threadIdx.x;
+    s_off[threadIdx.x] = ti < n ? off[ti] : INT64_MAX;
+    m = (int)(n - t0 < kChopRows ? n - t0 : kChopRows);
+    __syncthreads();
+  }
   if (r < n_steps) {
-    // last track whose first row is <= r (every track has >= 1 row)
-    int64_t lo = 0, hi = n - 1;
+    int lo = 0, hi = m - 1;
     while (lo < hi) {
-      const int64_t mid = (lo + hi + 1) >> 1;
-      if (off[mid] <= r) lo = mid;
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_off[mid] <= r) lo = mid;
       else hi = mid - 1;
     }
-    const float* in = tracks + lo * F;
+    const int64_t tr = t0 + lo;
+    const float* in = tracks + tr * F;
     for (int c = 0; c < F; ++c) row[c] = in[c];
     const ChopTrack t = chop_track(L, in);
     const float pf = (float)prec;
     int64_t nst = chop_steps(t.length, pf);
     if (nst == 0) nst = 1;
-    const int64_t k = r - off[lo];
+    const int64_t k = r - s_off[lo];
     const bool last = k == nst - 1;
     row[L.dE] = __fdiv_rn(in[L.dE] * pf, t.den);
     const double kp = (double)k * prec, k1p = prec * (double)(k + 1);
@@ -127,7 +159,7 @@ __global__ __launch_bounds__(kChopRows) void k_chop_fill(ChopLayout L, const flo
     for (int c = 0; c < 3; ++c) row[L.m[c]] = 0.5f * (row[L.s[c]] + row[L.e[c]]);
     if (src_ev) {
       // searchsorted(batch_event_ids, source_event_id) with a membership check (dataio.py:47-61)
-      const int64_t g = src_ev[lo];
+      const int64_t g = src_ev[tr];
       int64_t a = 0, b = n_batch;
       while (a < b) {
         const int64_t mid = (a + b) >> 1;
@@ -151,7 +183,13 @@ __global__ __launch_bounds__(kChopRows) void k_chop_fill(ChopLayout L, const flo
   const int64_t rows = n_rows - r0 < kChopRows ? n_rows - r0 : kChopRows;
   const int cnt = (int)rows * F;
   float* dst = out + r0 * F;
-  for (int j = threadIdx.x; j < cnt; j += kChopRows) dst[j] = tile[j];
+  if (V4) {  // out is 16-byte aligned and 256 * F floats are a whole number of float4
+    const int c4 = cnt >> 2;
+    for (int j = threadIdx.x; j < c4; j += kChopRows) reinterpret_cast<float4*>(dst)[j] = reinterpret_cast<const float4*>(tile)[j];
+    for (int j = (c4 << 2) + threadIdx.x; j < cnt; j += kChopRows) dst[j] = tile[j];
+  } else {
+    for (int j = threadIdx.x; j < cnt; j += kChopRows) dst[j] = tile[j];
+  }
 }
 
 }  // namespace lsim

@@ -705,6 +705,18 @@ int lsim_rbf_sums(const float* x, int64_t nx, const int32_t* lo, const int32_t* 
   return LSIM_OK;
 }
 
+int lsim_nearest(const float* x, int64_t nx, const int32_t* lo, const int32_t* hi, const float* y, int64_t ny,
+                 int64_t n_origin, int32_t* idx, float* dmin, void* stream) {
+  if (nx < 0 || ny < 0 || n_origin < 0 || n_origin > nx) return fail(LSIM_E_ARG, "bad sizes");
+  if (nx == 0) return LSIM_OK;
+  if (!x || !lo || !hi || !idx || !dmin || (ny > 0 && !y)) return fail(LSIM_E_ARG, "NULL argument");
+  if (nx >= ((int64_t)1 << 40) || ny >= ((int64_t)1 << 31)) return fail(LSIM_E_ARG, "too many points");
+  hipLaunchKernelGGL(k_nearest, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (const float4*)x, nx, lo, hi, (const float4*)y, n_origin, idx, dmin);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
 // ---- track chopping ----------------------------------------------------------------------------
 
 static int chop_layout(const lsim_chop_layout* l, ChopLayout& L) {
@@ -789,9 +801,15 @@ int lsim_chop_fill(const lsim_chop_layout* layout, const float* tracks, int64_t 
   if (n_rows == 0) return LSIM_OK;
   if (!out || !info || (n_steps > 0 && !ws)) return fail(LSIM_E_ARG, "NULL output, info or workspace");
   const size_t lds = (size_t)kChopRows * L.F * sizeof(float);
-  hipLaunchKernelGGL(k_chop_fill, dim3((unsigned)((n_rows + kChopRows - 1) / kChopRows)), dim3(kChopRows), lds,
-                     (hipStream_t)stream, L, tracks, n_tracks, precision, (const int64_t*)ws, n_steps, n_rows,
-                     src_events, batch_events, n_batch_events, n_local_events, out, info);
+  const dim3 grid((unsigned)((n_rows + kChopRows - 1) / kChopRows));
+  if (((uintptr_t)out & 15) == 0)
+    hipLaunchKernelGGL(k_chop_fill<true>, grid, dim3(kChopRows), lds, (hipStream_t)stream, L, tracks, n_tracks,
+                       precision, (const int64_t*)ws, n_steps, n_rows, src_events, batch_events, n_batch_events,
+                       n_local_events, out, info);
+  else
+    hipLaunchKernelGGL(k_chop_fill<false>, grid, dim3(kChopRows), lds, (hipStream_t)stream, L, tracks, n_tracks,
+                       precision, (const int64_t*)ws, n_steps, n_rows, src_events, batch_events, n_batch_events,
+                       n_local_events, out, info);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }

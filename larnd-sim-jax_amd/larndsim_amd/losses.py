@@ -64,7 +64,9 @@ def _kernel_rows(a, b, inv, c):
 
 
 def _pad4(a):
-    """(N, 3) -> contiguous (N, 4) float32 (fourth column zero)."""
+    """(N, 3) -> contiguous (N, 4) float32 (fourth column zero); (N, 4) -> contiguous float32."""
+    if a.shape[1] == 4:
+        return a.to(torch.float32).contiguous()
     out = torch.zeros((a.shape[0], 4), dtype=torch.float32, device=a.device)
     out[:, :3] = a
     return out
@@ -214,3 +216,183 @@ def mse_adc_blockwise(params, Q, x, y, z, ticks, hit_prob, event, ref_Q, ref_x, 
     charge_loss = ((tot - tot_ref) / (tot_ref + 1e-6)) ** 2
     aux = dict(charge_loss=charge_loss, mmd_loss_term=m)
     return m + lambda_Q * charge_loss, aux
+
+
+# --- chamfer_3d and nll_loss (losses_jax.py:195-249, 329-361) ---------------------------------
+
+def event_positions(x, y, z, event, offset):
+    """Event-local positions (fl(x + event * offset) - event * offset, y, z) in float32.
+
+    The reference stacks x + event * offset in float32, which for event >= 1 and offset 1e9
+    rounds x to a multiple of 64 cm; the rounded value is kept here, and the offset (exactly
+    representable, and the subtraction exact) is taken off again.  Differences between points of
+    one event, the only ones that matter, are therefore the reference's bit for bit, while the
+    gradients avoid the cancellation of 1e9-sized coordinates.  d/dx is 1, as in the reference.
+    """
+    off = event.to(torch.float32) * offset
+    xo = (x.to(torch.float32) + off) - off
+    return torch.stack((xo, y.to(torch.float32), z.to(torch.float32)), dim=-1)
+
+
+def _sorted_blocks(qe, ce):
+    """Stable event order of queries and candidates, and each sorted query's [lo, hi) of its event
+    among the sorted candidates."""
+    oq = torch.argsort(qe, stable=True)
+    oc = torch.argsort(ce, stable=True)
+    qs, cs = qe[oq].contiguous(), ce[oc].contiguous()
+    lo = torch.searchsorted(cs, qs, right=False).to(torch.int32)
+    hi = torch.searchsorted(cs, qs, right=True).to(torch.int32)
+    return oq, oc, qs, lo, hi
+
+
+def _nearest(pq, eq, pc, ec, origin):
+    """For each query point, the index (into pc) of its nearest candidate of the same event (first on
+    ties) and the squared distance, on the GPU (lsim_nearest).  With ``origin`` the queries of
+    event 0 also see the zero-weight padding point at the origin (index -1 when it wins)."""
+    import ctypes as C
+
+    from . import _lib
+
+    lib = _lib.load()
+    oq, oc, qs, lo, hi = _sorted_blocks(eq.to(torch.int64), ec.to(torch.int64))
+    q4, c4 = _pad4(pq[oq]), _pad4(pc[oc])
+    n0 = int((qs == 0).sum().item()) if origin else 0
+    idx = torch.empty(pq.shape[0], dtype=torch.int32, device=pq.device)
+    d = torch.empty(pq.shape[0], dtype=torch.float32, device=pq.device)
+    st = C.c_void_p(torch.cuda.current_stream(pq.device).cuda_stream)
+    _lib.check(lib.lsim_nearest(C.c_void_p(q4.data_ptr()), q4.shape[0], C.c_void_p(lo.data_ptr()),
+                                C.c_void_p(hi.data_ptr()), C.c_void_p(c4.data_ptr()), c4.shape[0], n0,
+                                C.c_void_p(idx.data_ptr()), C.c_void_p(d.data_ptr()), st))
+    i_sorted = idx.to(torch.int64)
+    i_orig = torch.where(i_sorted >= 0, oc[i_sorted.clamp(min=0)], torch.full_like(i_sorted, -1))
+    out_i = torch.empty_like(i_orig)
+    out_d = torch.empty_like(d)
+    out_i[oq] = i_orig
+    out_d[oq] = d
+    return out_i, out_d
+
+
+def _take_or_zero(v, i, m):
+    """v[i] where m, else 0 (v may be empty when m is all false)."""
+    if v.shape[0] == 0:
+        return torch.zeros(i.shape[0], dtype=v.dtype, device=v.device)
+    return torch.where(m, v[i.clamp(min=0)], 0.0)
+
+
+class _ChamferFn(torch.autograd.Function):
+    """chamfer_distance_3d_og (losses_jax.py:195-229) between the simulated hits a and the target
+    hits b, with points interacting only within their event (the reference's 1e9 cm event offset
+    puts every cross-event distance above its 1e4 cut).  Gradients flow to the simulated positions
+    and weights through the nearest pairs, with the argmins fixed, as reverse-mode AD does."""
+
+    @staticmethod
+    def forward(ctx, pa, wa, pb, wb, ea, eb, pad_a, pad_b):
+        if not pa.is_cuda:
+            raise RuntimeError("chamfer_3d runs on the GPU (lsim_nearest)")
+        ia, da = _nearest(pa, ea, pb, eb, pad_b)
+        ib, db = _nearest(pb, eb, pa, ea, pad_a)
+        wa64, wb64 = wa.to(torch.float64), wb.to(torch.float64)
+        ma = (ia >= 0) & (da < 1e4)
+        mb = (ib >= 0) & (db < 1e4)
+        wb_at = _take_or_zero(wb64, ia, ma)
+        wa_at = _take_or_zero(wa64, ib, mb)
+        la = torch.where(ma, da.to(torch.float64), 0.0) * wa64 * wb_at
+        lb = torch.where(mb, db.to(torch.float64), 0.0) * wb64 * wa_at
+        ctx.save_for_backward(pa, pb, wa64, wb64, ia, ib, da, db, ma, mb, wb_at)
+        ctx.wa_dtype = wa.dtype
+        return la.sum() + lb.sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        pa, pb, wa, wb, ia, ib, da, db, ma, mb, wb_at = ctx.saved_tensors
+        pa64, pb64 = pa.to(torch.float64), pb.to(torch.float64)
+        # a -> b: d(d_a wa wb*) = 2 (pa - pb*) wa wb* dpa + d_a wb* dwa
+        diff_a = pa64 - pb64[ia.clamp(min=0)] if pb64.shape[0] else torch.zeros_like(pa64)
+        g_pa = torch.where(ma[:, None], 2.0 * diff_a * (wa * wb_at)[:, None], 0.0)
+        g_wa = torch.where(ma, da.to(torch.float64) * wb_at, 0.0)
+        # b -> a: the nearest a* of each b gets 2 (pa* - pb) wb wa* and d_b wb
+        sel = torch.nonzero(mb).squeeze(1)
+        if sel.numel():
+            a_star = ib[sel]
+            diff_b = pa64[a_star] - pb64[sel]
+            g_pa = g_pa.index_add(0, a_star, 2.0 * diff_b * (wb[sel] * wa[a_star])[:, None])
+            g_wa = g_wa.index_add(0, a_star, db[sel].to(torch.float64) * wb[sel])
+        return (g * g_pa).to(pa.dtype), (g * g_wa).to(ctx.wa_dtype), None, None, None, None, None, None
+
+
+def chamfer_3d(params, Q, x, y, z, ticks, hit_prob, event, ref_Q, ref_x, ref_y, ref_z, ref_ticks, ref_hit_prob,
+               ref_event, adc_norm=10., match_z=False):
+    """Weighted chamfer distance between simulated and target hits (losses_jax.py:231-249).
+
+    Positions are (x + event * 1e9, y, z) in float32 (kept event-local, see ``event_positions``)
+    and weights Q / adc_norm, as in the reference;
+    the padding that ``pad_size`` adds there (zero-weight points at the origin) is honoured: an
+    event-0 hit whose nearest point is such a padding point contributes nothing.  The pairwise
+    minimum runs on the GPU per event (``lsim_nearest``).  Returns (loss, {})."""
+    from .sim import pad_size
+
+    n_ref, n_cur = int(ref_x.shape[0]), int(x.shape[0])
+    new_ref, new_cur = pad_size((n_ref, n_cur), "chamfer_3d")
+    pa = event_positions(x, y, z, event, 1e9)
+    pb = event_positions(ref_x, ref_y, ref_z, ref_event, 1e9)
+    wa = Q.to(torch.float32) / adc_norm
+    wb = ref_Q.to(torch.float32) / adc_norm
+    loss = _ChamferFn.apply(pa, wa, pb, wb, event.to(torch.int64), ref_event.to(torch.int64), new_cur > n_cur,
+                            new_ref > n_ref)
+    return loss, dict()
+
+
+class _NLLFn(torch.autograd.Function):
+    """-sum_j log(lambda_j + 1e-10) [ref_p_j > 0.5] with lambda_j = sum_i p_i K(c_i, r_j), the 4-D
+    Gaussian-kernel intensity of nll_loss (losses_jax.py:329-361), summed within events (the 1e9 cm
+    event offset makes every cross-event kernel value exactly 0 in float32).  Kernel values in
+    float32 and sums in float64 on the GPU (lsim_rbf_sums); gradients to the simulated points and
+    probabilities."""
+
+    @staticmethod
+    def forward(ctx, c4, p, r4, rp, ec, er, sigma):
+        if not c4.is_cuda:
+            raise RuntimeError("nll_loss runs on the GPU (lsim_rbf_sums)")
+        coef = torch.zeros((c4.shape[0], 4), dtype=torch.float64, device=c4.device)
+        coef[:, 0] = p.to(torch.float64)
+        lam = _block_sums(r4, er, c4, ec, coef, sigma)[:, 0]
+        m = (rp > 0.5).to(torch.float64)
+        term1 = (torch.log(lam + 1e-10) * m).sum()
+        ctx.save_for_backward(c4, p, r4, ec, er, lam, m)
+        ctx.sigma = sigma
+        return -term1
+
+    @staticmethod
+    def backward(ctx, g):
+        c4, p, r4, ec, er, lam, m = ctx.saved_tensors
+        s2 = ctx.sigma * ctx.sigma
+        gj = m / (lam + 1e-10)
+        r64 = r4.to(torch.float64)
+        coef = torch.stack((gj, gj * r64[:, 0], gj * r64[:, 1], gj * r64[:, 2]), dim=1)
+        S = _block_sums(c4, ec, r4, er, coef, ctx.sigma)
+        coef2 = torch.zeros_like(coef)
+        coef2[:, 0] = gj * r64[:, 3]
+        S3 = _block_sums(c4, ec, r4, er, coef2, ctx.sigma)[:, 0]
+        c64, p64 = c4.to(torch.float64), p.to(torch.float64)
+        # d(-term1)/dp_i = -(K g)_i ; d(-term1)/dc_i = (p_i / s^2) (c_i (K g)_i - (K g r)_i)
+        g_p = -S[:, 0]
+        Sr = torch.cat((S[:, 1:], S3[:, None]), dim=1)
+        g_c = (p64 / s2)[:, None] * (c64 * S[:, :1] - Sr)
+        return (g * g_c).to(c4.dtype), (g * g_p).to(p.dtype), None, None, None, None, None
+
+
+def nll_loss(params, Q, x, y, z, ticks, hit_prob, event, ref_Q, ref_x, ref_y, ref_z, ref_ticks, ref_hit_prob,
+             ref_event, adc_norm=10., sigma=1., match_z=False):
+    """Poisson-process negative log-likelihood of the target hits under the simulated hits
+    (losses_jax.py:329-361): sum(hit_prob) - sum_j log(lambda_j + 1e-10) over target hits with
+    hit_prob > 0.5, lambda_j = sum_i hit_prob_i exp(-|c_i - r_j|^2 / 2 sigma^2) over the 4-D points
+    (x + event 1e9, y, z, Q / adc_norm).  Returns (loss, {})."""
+    from .sim import pad_size
+
+    pad_size((int(ref_x.shape[0]), int(x.shape[0])), "nll_loss")  # the reference's padding history
+    c4 = torch.cat((event_positions(x, y, z, event, 1e9), (Q.to(torch.float32) / adc_norm)[:, None]), dim=1)
+    r4 = torch.cat((event_positions(ref_x, ref_y, ref_z, ref_event, 1e9),
+                    (ref_Q.to(torch.float32) / adc_norm)[:, None]), dim=1)
+    p = hit_prob.to(torch.float32)
+    t1 = _NLLFn.apply(c4, p, r4, ref_hit_prob, event.to(torch.int64), ref_event.to(torch.int64), float(sigma))
+    return p.to(torch.float64).sum() + t1, dict()
