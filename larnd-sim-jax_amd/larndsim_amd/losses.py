@@ -206,11 +206,13 @@ def mmd_blockwise(x, y, px, py, sigma, ex, ey, chunk=8192):
 def mse_adc_blockwise(params, Q, x, y, z, ticks, hit_prob, event, ref_Q, ref_x, ref_y, ref_z, ref_ticks, ref_hit_prob,
                       ref_event, sigma=1, lambda_Q=1):
     """``mse_adc`` (losses_jax.py:58-80) with the event-blocked MMD: the same loss (the reference
-    separates events by offsetting x by event * 1e5 cm, which zeroes every cross-event kernel)."""
+    separates events by offsetting x by event * 1e5 cm, which zeroes every cross-event kernel).
+    Positions keep the reference's float32 rounding of x + event * 1e5 (event-local, see
+    ``event_positions``)."""
     w_ref = (ref_Q * ref_hit_prob).to(torch.float64)
     w = (Q * hit_prob).to(torch.float64)
-    s = torch.stack((x, y, z), dim=-1).to(torch.float64)
-    ref_s = torch.stack((ref_x, ref_y, ref_z), dim=-1).to(torch.float64)
+    s = event_positions(x, y, z, event, 1e5).to(torch.float64)
+    ref_s = event_positions(ref_x, ref_y, ref_z, ref_event, 1e5).to(torch.float64)
     m = mmd_blockwise(s, ref_s, w, w_ref, sigma, event.to(torch.int64), ref_event.to(torch.int64))
     tot_ref, tot = w_ref.sum(), w.sum()
     charge_loss = ((tot - tot_ref) / (tot_ref + 1e-6)) ** 2
@@ -396,3 +398,13 @@ def nll_loss(params, Q, x, y, z, ticks, hit_prob, event, ref_Q, ref_x, ref_y, re
     p = hit_prob.to(torch.float32)
     t1 = _NLLFn.apply(c4, p, r4, ref_hit_prob, event.to(torch.int64), ref_event.to(torch.int64), float(sigma))
     return p.to(torch.float64).sum() + t1, dict()
+
+
+def loss_functions(adc_norm=10.0, match_z=False):
+    """The named fit losses of the reference's driver (optimize/fit_params.py:302-312) that run
+    here, as (function, keyword arguments) for ``GenericLossStrategy(fn, **kw)``."""
+    return {
+        "mse_adc": (mse_adc_blockwise, {}),
+        "chamfer_3d": (chamfer_3d, {"adc_norm": adc_norm, "match_z": match_z}),
+        "nll": (nll_loss, {"adc_norm": adc_norm, "sigma": 1.0}),
+    }
