@@ -13,6 +13,7 @@ Adam fit of the 7 parameters on those 32 events (steps/s, parameter recovery), a
 baseline (the float32 NumPy restatement of the reference algorithm on a bounded sample, on the
 host cores).
 
+``--prob-fee-only`` runs only the probabilistic readout on the 200k-segment event's rows.
 ``--chop-only`` runs only the track-chopping leg (350k track rows -> ~3.5M segment rows).
 ``--batched-bwd-only`` runs only the 32-event forward+backward graph replays (for kernel profiles).
 ``--copy-calibrate`` only runs a plain 256 MiB device copy five times: profiled with the same
@@ -183,6 +184,48 @@ def chop_leg(dev, steps, n_tracks=350_000, precision=0.1, cpu=True):
     return leg
 
 
+def prob_fee_leg(params, wfs, steps, cpu=True, cpu_rows=16):
+    """The probabilistic readout (beam search of 20 paths, 10 hit indices, reset noise 900 e-) on
+    the waveform rows of the 200k-segment event, inputs resident; timed with events on torch's
+    current stream.  CPU baseline: the float64 NumPy restatement on the first cpu_rows rows."""
+    import numpy as np
+    import torch
+
+    from larndsim_amd.fee import get_adc_values_average_noise_vmap
+
+    pp = params.replace(RESET_NOISE_CHARGE=900.0)
+    for _ in range(2):
+        get_adc_values_average_noise_vmap(pp, wfs)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        lp, q = get_adc_values_average_noise_vmap(pp, wfs)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    P, T = wfs.shape
+    V, S = int(pp.fee_paths_scaling), int(pp.MAX_ADC_VALUES)
+    active = int((lp.amax(dim=2) > -999.0).any(dim=0).sum())
+    leg = dict(rows=P, ticks=T, paths=V, hit_indices=S, steps_run=active, ms=ms, rows_per_s=P / (ms * 1e-3),
+               path_ticks_per_s=P * active * V * (T - 1) / (ms * 1e-3),
+               out_bytes=2 * P * S * (T - 1) * 4, out_gbs=2 * P * S * (T - 1) * 4 / (ms * 1e-3) / 1e9)
+    if cpu:
+        sys.path.insert(0, ROOT)
+        from threadpoolctl import threadpool_limits
+
+        from oracle import probfee as PF
+
+        w = wfs[:cpu_rows].cpu().numpy()
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            PF.prob_fee(w, pp.t_sampling, V, S, pp.adc_interval, pp.DISCRIMINATION_THRESHOLD, pp.RESET_NOISE_CHARGE)
+            dt = time.perf_counter() - t0
+        leg["cpu_baseline"] = dict(value=cpu_rows / dt, unit="rows/s", cores=1, kind="port",
+                                   sample=f"first {cpu_rows} rows, float64 NumPy restatement, 1 thread, {dt:.2f} s")
+    return leg
+
+
 def batched_fwd_bwd(simr, params, dev, steps, fd_baseline=True):
     """32 uneven events (20k-200k segments): forward + backward replayed as one hipGraph, waveform
     rows recomputed (not stored), against the per-event forward-then-finite-difference gradient."""
@@ -269,6 +312,9 @@ def main():
     ap.add_argument("--copy-calibrate", action="store_true")
     ap.add_argument("--no-chop", action="store_true", help="skip the track-chopping leg")
     ap.add_argument("--chop-only", action="store_true", help="profiling: only the track-chopping leg")
+    ap.add_argument("--no-prob-fee", action="store_true", help="skip the probabilistic readout leg")
+    ap.add_argument("--prob-fee-only", action="store_true",
+                    help="profiling: only the probabilistic readout on the 200k-segment event's rows")
     ap.add_argument("--batched-bwd-only", action="store_true",
                     help="profiling: only the 32-event batched forward+backward graph replays (no baseline)")
     a = ap.parse_args()
@@ -301,6 +347,11 @@ def main():
         if a.waves:
             S.set_row_waves(a.waves)
         print(json.dumps(batched_fwd_bwd(S.Simulator(params, lut), params, dev, a.steps, fd_baseline=False)))
+        return
+    if a.prob_fee_only:
+        tracks, fields = synthetic.workload("c2")
+        st = S.Simulator(params, lut).forward(params, torch.from_numpy(tracks).to(dev), fields, n_events=1)
+        print(json.dumps(prob_fee_leg(params, st.out["wfs"][: st.max_rows].contiguous(), a.steps, cpu=not a.no_cpu)))
         return
     log("[bench] generating the 200k-segment event")
     tracks, fields = synthetic.workload("c2")
@@ -409,6 +460,15 @@ def main():
     # ---- batched forward + backward: 32 uneven events, one graph replay, vs per-event FD ----
     if not a.no_batched_bwd:
         extra.update(batched_fwd_bwd(simr, params, dev, a.steps))
+
+    # ---- probabilistic readout of the same event's rows ----
+    if not a.no_prob_fee:
+        log("[bench] probabilistic readout")
+        wst = simr.forward(params, trk, fields, n_events=1)
+        extra["prob_fee"] = prob_fee_leg(params, wst.out["wfs"][: wst.max_rows].contiguous(), max(3, a.steps // 4),
+                                         cpu=not a.no_cpu)
+        log(f"[bench] prob fee {extra['prob_fee']['rows']} rows: {extra['prob_fee']['ms']:.2f} ms")
+        del wst
 
     # ---- track chopping (the data step before the path) ----
     if not a.no_chop:

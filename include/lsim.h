@@ -240,6 +240,23 @@ int lsim_chop_fill(const lsim_chop_layout *layout, const float *tracks, int64_t 
                    const int64_t *batch_events, int64_t n_batch_events, int64_t n_local_events, float *out,
                    int64_t *info, void *stream);
 
+/* ---- Probabilistic self-trigger readout (the step after the path in the llhd fits) ------------- */
+
+/* Workspace bytes of lsim_prob_fee. */
+int lsim_prob_fee_query(int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, size_t *ws_bytes);
+
+/* get_adc_values_average_noise_vmap (src/larndsim/fee_jax.py:334-461): the beam search of n_paths
+ * reset paths per waveform row with Gaussian reset noise noise_sigma (e-), n_hits steps.
+ * wfs [n_rows][n_ticks] float32 (current per tick; charge = wfs * t_sampling).  Outputs
+ * log_prob and charge [n_rows][n_hits][n_ticks - 1] float32: the log probability that hit s
+ * starts at tick t and its expected charge (e-).  interval = round((3 + ADC_HOLD_DELAY)
+ * CLOCK_CYCLE / t_sampling) ticks.  As in the reference, all rows stop together once no row's
+ * paths keep a total probability above stop_threshold; stopped steps hold -1000 and 0.
+ * n_ticks <= 2048, n_paths <= 32.  Forward only. */
+int lsim_prob_fee(const float *wfs, int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, int32_t interval,
+                  float threshold, float noise_sigma, float t_sampling, float stop_threshold, void *ws,
+                  size_t ws_bytes, float *log_prob, float *charge, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@
 #include "backward.h"
 #include "chop.h"
 #include "loss.h"
+#include "probfee.h"
 
 using namespace lsim;
 
@@ -811,6 +812,61 @@ int lsim_chop_fill(const lsim_chop_layout* layout, const float* tracks, int64_t 
                        precision, (const int64_t*)ws, n_steps, n_rows, src_events, batch_events, n_batch_events,
                        n_local_events, out, info);
   HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+// ---- probabilistic readout ---------------------------------------------------------------------
+
+static size_t pf_flags_bytes(int S) { return (((size_t)(S + 1) * sizeof(int) + 255) / 256) * 256; }
+
+static size_t pf_ws_bytes(int64_t P, int T, int V, int S) {
+  return pf_flags_bytes(S) + (size_t)P * 4 * T * sizeof(float) + (size_t)2 * P * V * sizeof(float);
+}
+
+int lsim_prob_fee_query(int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, size_t* ws_bytes) {
+  if (n_rows < 0 || n_ticks < 2 || n_paths < 1 || n_hits < 1 || !ws_bytes) return fail(LSIM_E_ARG, "bad arguments");
+  *ws_bytes = pf_ws_bytes(n_rows, n_ticks, n_paths, n_hits);
+  return LSIM_OK;
+}
+
+int lsim_prob_fee(const float* wfs, int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, int32_t interval,
+                  float threshold, float noise_sigma, float t_sampling, float stop_threshold, void* ws,
+                  size_t ws_bytes, float* log_prob, float* charge, void* stream) {
+  if (n_rows < 0) return fail(LSIM_E_ARG, "n_rows must be >= 0");
+  if (n_ticks < 2 || n_ticks > kPfMaxTicks) return fail(LSIM_E_ARG, "n_ticks must be in 2 .. 2048");
+  if (n_paths < 1 || n_paths > kPfMaxPaths || n_paths > n_ticks - 1)
+    return fail(LSIM_E_ARG, "n_paths must be in 1 .. min(32, n_ticks - 1)");
+  if (n_hits < 1 || n_hits > 64) return fail(LSIM_E_ARG, "n_hits must be in 1 .. 64");
+  if (interval < 0) return fail(LSIM_E_ARG, "interval must be >= 0");
+  if (!(noise_sigma > 0.f) || !(t_sampling > 0.f) || !(stop_threshold > 0.f))
+    return fail(LSIM_E_ARG, "noise_sigma, t_sampling and stop_threshold must be > 0");
+  if (n_rows >= ((int64_t)1 << 31)) return fail(LSIM_E_ARG, "too many rows");
+  if (ws_bytes < pf_ws_bytes(n_rows, n_ticks, n_paths, n_hits)) return fail(LSIM_E_SPACE, "workspace too small");
+  if (n_rows == 0) return LSIM_OK;
+  if (!wfs || !ws || !log_prob || !charge) return fail(LSIM_E_ARG, "NULL argument");
+  PfDev d;
+  d.T = n_ticks;
+  d.V = n_paths;
+  d.S = n_hits;
+  d.interval = interval;
+  d.theta = threshold;
+  d.zs = 1.f / noise_sigma;
+  d.ts = t_sampling;
+  d.log_stop = logf(stop_threshold);
+  hipStream_t st = (hipStream_t)stream;
+  int* flags = (int*)ws;
+  float* tab = (float*)((char*)ws + pf_flags_bytes(n_hits));
+  float* state = tab + (size_t)n_rows * 4 * n_ticks;
+  hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, st, flags, (int64_t)(n_hits + 1), 0);
+  hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, st, flags, (int64_t)1, 1);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_pf_prep, dim3((unsigned)n_rows), dim3(kPfThreads), 0, st, d, wfs, n_rows, tab);
+  HIP_TRY(hipGetLastError());
+  for (int s = 0; s < n_hits; ++s) {
+    hipLaunchKernelGGL(k_pf_step, dim3((unsigned)n_rows), dim3(kPfThreads), 0, st, d, s, (const float*)tab, n_rows,
+                       state, flags, log_prob, charge);
+    HIP_TRY(hipGetLastError());
+  }
   return LSIM_OK;
 }
 

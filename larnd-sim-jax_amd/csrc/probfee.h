@@ -1,0 +1,366 @@
+// Probabilistic self-trigger readout (fee_jax.py:334-461 _find_one_hit_step and
+// get_adc_values_average_noise_vmap): a beam search of V charge-reset paths per pixel row over
+// the tick grid, with Gaussian reset noise sigma, giving per hit index s and tick t the log
+// probability that hit s starts at t and its expected charge.
+//
+// Per row, with q_sum the cumulative charge (T ticks), c_v and p_v the reset charge and log
+// probability of path v, sh(t) = min(t + interval + 1, T - 1) and z(x) = (x - threshold) / sigma:
+//   log_guess[v,t] = logdiffPhi(z(CM[t+1] - c_v), z(CM[t] - c_v)),  CM = cumulative max of q_sum
+//                    (cummax(q_sum - c) = cummax(q_sum) - c, so one CM serves every path);
+//   log_event[v,t] = min(logdiffPhi(z(q_sum[sh] - c_v), z(q_sum[t] - c_v)), log_guess), then
+//                    soft-clamped at -1000 and soft-masked by the expected charge
+//                    esp[t] = q_sum[sh] + threshold - (q_sum[t+1] + q_sum[t]) / 2;
+//   out_logp[s,t]  = logsumexp_v(log_event + p_v),  out_charge[s,t] = esp[t];
+//   total[t]       = logsumexp_v(log_guess + p_v);
+//   select[t]      = LF[t] + total[t], LF[t] = log Phi(z(RCM[e(t)] - q_sum[sh(t)+1])), RCM the
+//                    reverse cumulative max, e(t) = min(sh(t) + interval + 1, T - 1) (the path
+//                    charge cancels in this term, so it is one value per tick);
+//   the V ticks of largest select (lower tick first on ties) become the next paths:
+//   p'_k = total[t_k], c'_k = q_sum[min(sh(t_k) + 1, T - 1)].
+// The reference stops all rows at once when no row has logsumexp(p') > log(stop) any more; the
+// step launches read and set that flag in device memory, and stopped steps write -1000 / 0.
+// Rows are independent: one workgroup per row and step, ticks in contiguous chunks per thread,
+// the row's tables in LDS, the top-V selection by V rounds of a block arg-max.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lsim {
+
+constexpr int kPfThreads = 256;
+constexpr int kPfMaxTicks = 2048;
+constexpr int kPfMaxPaths = 32;
+constexpr int kPfChunk = (kPfMaxTicks + kPfThreads - 1) / kPfThreads;  // ticks per thread
+
+struct PfDev {
+  int T;          // ticks per row
+  int V;          // paths kept (fee_paths_scaling)
+  int S;          // hit indices (MAX_ADC_VALUES)
+  int interval;   // ADC hold interval in ticks
+  float theta;    // discrimination threshold (e-)
+  float zs;       // 1 / reset-noise sigma
+  float ts;       // t_sampling
+  float log_stop; // log of the stop threshold
+};
+
+// log Phi(x) in float32 from one scaled complementary error function, u = |x| / sqrt 2:
+// x < 0: -u^2 + log(erfcx(u) / 2) (accurate far into the tail); x >= 0: log1p(-exp(-u^2) erfcx(u) / 2).
+// No branch on the sign around the expensive part, so lanes of either sign share one erfcx.
+__device__ __forceinline__ float pf_log_ndtr(float x) {
+  const float u = fabsf(x) * 0.70710678118654752f;
+  const float ex = erfcxf(u);
+  const float u2 = u * u;
+  return x < 0.f ? -u2 + logf(0.5f * ex) : log1pf(-0.5f * expf(-u2) * ex);
+}
+
+__device__ __forceinline__ float pf_softplus(float x) { return fmaxf(x, 0.f) + log1pf(expf(-fabsf(x))); }
+
+// lo + softplus((x - lo) s) / s (fee_jax.py:13-20)
+__device__ __forceinline__ float pf_soft_max(float x, float lo, float s) { return lo + pf_softplus((x - lo) * s) / s; }
+
+// sigmoid(c s) t + (1 - sigmoid(c s)) f (fee_jax.py:22-30)
+__device__ __forceinline__ float pf_soft_where(float c, float t, float f, float s) {
+  const float w = 1.f / (1.f + expf(-(c * s)));
+  return w * t + (1.f - w) * f;
+}
+
+// log(Phi(a) - Phi(b)) with the reference's smooth clamps (fee_jax.py:32-54), from la = log Phi(a)
+// and lb = log Phi(b).
+__device__ __forceinline__ float pf_log_diff(float a, float b, float la, float lb) {
+  const float sd = a > b ? lb - la : -1.f;
+  float ne = -expm1f(sd);
+  ne = pf_soft_max(ne, 1e-30f, 1e10f);
+  float lt = logf(ne);
+  lt = pf_soft_max(lt, -100.f, 10.f);
+  return pf_soft_where(a - b, la + lt, -1000.f, 1000.f);
+}
+
+// Running logsumexp: (m, sum) with sum = sum_i exp(x_i - m).
+__device__ __forceinline__ void pf_lse_add(float& m, float& sum, float x) {
+  if (x > m) {
+    sum = sum * expf(m - x) + 1.f;
+    m = x;
+  } else {
+    sum += expf(x - m);
+  }
+}
+
+// Block-wide inclusive scan (sum, double) of one value per thread; returns the inclusive prefix.
+__device__ __forceinline__ double pf_block_scan(double v, double* sh) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  double inc = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double u = __shfl_up(inc, d, kWave);
+    if (lane >= d) inc += u;
+  }
+  if (lane == kWave - 1) sh[wid] = inc;
+  __syncthreads();
+  double base = 0.0;
+  for (int w = 0; w < wid; ++w) base += sh[w];
+  __syncthreads();
+  return base + inc;
+}
+
+// Per row: q_sum, its cumulative max, LF and esp (all independent of the step).
+// tab layout per row: [q_sum T][CM T][LF T-1 (+1 pad)][esp T-1 (+1 pad)].
+__global__ __launch_bounds__(kPfThreads) void k_pf_prep(PfDev d, const float* __restrict__ wfs, int64_t P,
+                                                       float* __restrict__ tab) {
+  __shared__ float s_q[kPfMaxTicks];
+  __shared__ float s_r[kPfMaxTicks];
+  __shared__ double s_w[kPfThreads / kWave];
+  __shared__ float s_wm[kPfThreads / kWave];
+  const int64_t row = blockIdx.x;
+  const int T = d.T;
+  const float* w = wfs + row * T;
+  const int t0 = threadIdx.x * kPfChunk;
+  // cumulative sum of q = wfs * t_sampling (double prefix, rounded once)
+  double loc[kPfChunk];
+  double run = 0.0;
+#pragma unroll
+  for (int k = 0; k < kPfChunk; ++k) {
+    const int t = t0 + k;
+    const double q = t < T ? (double)(w[t] * d.ts) : 0.0;
+    run += q;
+    loc[k] = run;
+  }
+  const double base = pf_block_scan(run, s_w) - run;
+#pragma unroll
+  for (int k = 0; k < kPfChunk; ++k)
+    if (t0 + k < T) s_q[t0 + k] = (float)(base + loc[k]);
+  __syncthreads();
+  float* qs = tab + row * 4 * (int64_t)T;
+  float* cm = qs + T;
+  float* lf = cm + T;
+  float* esp = lf + T;
+  // forward and reverse cumulative max: chunk maxima, exclusive prefix / suffix maxima over
+  // threads (wave shuffles, then the wave totals in LDS), then a walk over the chunk
+  const float ninf = -__builtin_inff();
+  float m = ninf;
+#pragma unroll
+  for (int k = 0; k < kPfChunk; ++k)
+    if (t0 + k < T) m = fmaxf(m, s_q[t0 + k]);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  float inc = m;
+#pragma unroll
+  for (int dd = 1; dd < kWave; dd <<= 1) {
+    const float u = __shfl_up(inc, dd, kWave);
+    if (lane >= dd) inc = fmaxf(inc, u);
+  }
+  const float up1 = __shfl_up(inc, 1, kWave);
+  if (lane == kWave - 1) s_wm[wid] = inc;
+  __syncthreads();
+  float excl_f = lane > 0 ? up1 : ninf;
+  for (int ww = 0; ww < wid; ++ww) excl_f = fmaxf(excl_f, s_wm[ww]);
+  __syncthreads();
+  float incr = m;
+#pragma unroll
+  for (int dd = 1; dd < kWave; dd <<= 1) {
+    const float u = __shfl_down(incr, dd, kWave);
+    if (lane + dd < kWave) incr = fmaxf(incr, u);
+  }
+  const float dn1 = __shfl_down(incr, 1, kWave);
+  if (lane == 0) s_wm[wid] = incr;
+  __syncthreads();
+  float excl_r = lane + 1 < kWave ? dn1 : ninf;
+  for (int ww = wid + 1; ww < kPfThreads / kWave; ++ww) excl_r = fmaxf(excl_r, s_wm[ww]);
+  float cf = excl_f;
+#pragma unroll
+  for (int k = 0; k < kPfChunk; ++k) {
+    const int t = t0 + k;
+    if (t < T) {
+      cf = fmaxf(cf, s_q[t]);
+      cm[t] = cf;
+    }
+  }
+  float cr = excl_r;
+#pragma unroll
+  for (int k = kPfChunk - 1; k >= 0; --k) {
+    const int t = t0 + k;
+    if (t < T) {
+      cr = fmaxf(cr, s_q[t]);
+      s_r[t] = cr;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPfChunk; ++k) {
+    const int t = t0 + k;
+    if (t < T) qs[t] = s_q[t];
+    if (t < T - 1) {
+      const int sh = min(t + d.interval + 1, T - 1);
+      const int nx = min(sh + 1, T - 1);
+      const int e = min(sh + d.interval + 1, T - 1);
+      lf[t] = pf_log_ndtr((s_r[e] - s_q[nx] - d.theta) * d.zs);
+      esp[t] = s_q[sh] + d.theta - 0.5f * (s_q[t + 1] + s_q[t]);
+    }
+  }
+}
+
+// One beam-search step for every row.  state: [P][V] reset charges then [P][V] log probabilities.
+// flags[s] != 0: step s runs; a row whose new paths keep probability sets flags[s + 1].
+__global__ __launch_bounds__(kPfThreads) void k_pf_step(PfDev d, int s, const float* __restrict__ tab, int64_t P,
+                                                       float* __restrict__ state, int* __restrict__ flags,
+                                                       float* __restrict__ out_logp, float* __restrict__ out_q) {
+  __shared__ float s_q[kPfMaxTicks];
+  __shared__ float s_cm[kPfMaxTicks];
+  __shared__ float s_tot[kPfMaxTicks];
+  __shared__ float s_sel[kPfMaxTicks];
+  __shared__ float s_c[kPfMaxPaths], s_p[kPfMaxPaths];
+  __shared__ float s_bv[kPfThreads / kWave];
+  __shared__ int s_bi[kPfThreads / kWave];
+  __shared__ int s_top[kPfMaxPaths];
+  const int64_t row = blockIdx.x;
+  const int T = d.T, V = d.V, n = T - 1;
+  float* ol = out_logp + (row * d.S + s) * (int64_t)n;
+  float* oq = out_q + (row * d.S + s) * (int64_t)n;
+  if (!flags[s]) {  // every row has stopped: the reference's inactive branch
+    for (int t = threadIdx.x; t < n; t += kPfThreads) {
+      ol[t] = -1000.f;
+      oq[t] = 0.f;
+    }
+    return;
+  }
+  const float* qs = tab + row * 4 * (int64_t)T;
+  const float* cm = qs + T;
+  const float* lf = cm + T;
+  const float* esp = lf + T;
+  for (int t = threadIdx.x; t < T; t += kPfThreads) {
+    s_q[t] = qs[t];
+    s_cm[t] = cm[t];
+  }
+  float* st_c = state + row * (int64_t)V;
+  float* st_p = state + (P + row) * (int64_t)V;
+  __shared__ int s_nv;
+  if (threadIdx.x < V) {
+    s_c[threadIdx.x] = s == 0 ? 0.f : st_c[threadIdx.x];
+    s_p[threadIdx.x] = s == 0 ? (threadIdx.x == 0 ? 0.f : -1000.f) : st_p[threadIdx.x];
+  }
+  __syncthreads();
+  // Paths with the same reset charge have the same log_event and log_guess at every tick, so their
+  // terms differ only by p_v: merge them into one path with p = logsumexp of theirs (in path
+  // order).  At step 0 all V start paths share charge 0 and collapse into one.
+  if (threadIdx.x == 0) {
+    int nv = 0;
+    for (int v = 0; v < V; ++v) {
+      int u = 0;
+      while (u < nv && s_c[u] != s_c[v]) ++u;
+      if (u == nv) {
+        s_c[nv] = s_c[v];
+        s_p[nv] = s_p[v];
+        ++nv;
+      } else {
+        const float a = s_p[u], b = s_p[v], m = fmaxf(a, b);
+        s_p[u] = m + log1pf(expf(fminf(a, b) - m));
+      }
+    }
+    s_nv = nv;
+  }
+  __syncthreads();
+  const int NV = s_nv;
+  const int t0 = threadIdx.x * kPfChunk;
+  float hmax[kPfChunk], hsum[kPfChunk], gmax[kPfChunk], gsum[kPfChunk];
+#pragma unroll
+  for (int k = 0; k < kPfChunk; ++k) {
+    hmax[k] = -__builtin_inff();
+    gmax[k] = -__builtin_inff();
+    hsum[k] = 0.f;
+    gsum[k] = 0.f;
+  }
+  float espk[kPfChunk];
+#pragma unroll
+  for (int k = 0; k < kPfChunk; ++k) espk[k] = t0 + k < n ? esp[t0 + k] : 0.f;
+  // one pass over the (merged) paths with running logsumexp accumulators (max and shifted sum)
+  for (int v = 0; v < NV; ++v) {
+    const float c = s_c[v], pv = s_p[v];
+    float b1 = (s_cm[min(t0, T - 1)] - c - d.theta) * d.zs;
+    float lb1 = pf_log_ndtr(b1);
+#pragma unroll
+    for (int k = 0; k < kPfChunk; ++k) {
+      const int t = t0 + k;
+      if (t < n) {
+        const float a1 = (s_cm[t + 1] - c - d.theta) * d.zs;
+        const float la1 = pf_log_ndtr(a1);
+        const float lg = pf_log_diff(a1, b1, la1, lb1);
+        const int sh = min(t + d.interval + 1, T - 1);
+        const float a2 = (s_q[sh] - c - d.theta) * d.zs, b2 = (s_q[t] - c - d.theta) * d.zs;
+        float le = pf_log_diff(a2, b2, pf_log_ndtr(a2), pf_log_ndtr(b2));
+        le = fminf(le, lg);
+        le = pf_soft_max(le, -1000.f, 1.f);
+        le = pf_soft_where(espk[k] - d.theta, le, -1000.f, 10.f);
+        pf_lse_add(hmax[k], hsum[k], le + pv);
+        pf_lse_add(gmax[k], gsum[k], lg + pv);
+        b1 = a1;
+        lb1 = la1;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPfChunk; ++k) {
+    const int t = t0 + k;
+    if (t < n) {
+      ol[t] = hmax[k] + logf(hsum[k]);
+      oq[t] = espk[k];
+      const float tot = gmax[k] + logf(gsum[k]);
+      s_tot[t] = tot;
+      s_sel[t] = lf[t] + tot;
+    }
+  }
+  __syncthreads();
+  // top V ticks of select, lower tick first on ties
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  for (int r = 0; r < V; ++r) {
+    float bv = -__builtin_inff();
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < kPfChunk; ++k) {
+      const int t = t0 + k;
+      if (t < n && (s_sel[t] > bv || (s_sel[t] == bv && t < bi))) {
+        bv = s_sel[t];
+        bi = t;
+      }
+    }
+#pragma unroll
+    for (int dd = kWave / 2; dd > 0; dd >>= 1) {
+      const float ov = __shfl_xor(bv, dd, kWave);
+      const int oi = __shfl_xor(bi, dd, kWave);
+      if (ov > bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      s_bv[wid] = bv;
+      s_bi[wid] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float v0 = s_bv[0];
+      int i0 = s_bi[0];
+      for (int w = 1; w < kPfThreads / kWave; ++w)
+        if (s_bv[w] > v0 || (s_bv[w] == v0 && s_bi[w] < i0)) {
+          v0 = s_bv[w];
+          i0 = s_bi[w];
+        }
+      if (i0 < 0 || i0 >= n) i0 = 0;  // all -inf (cannot happen for finite inputs)
+      s_top[r] = i0;
+      s_sel[i0] = -__builtin_inff();
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < V) {
+    const int t = s_top[threadIdx.x];
+    const int nx = min(min(t + d.interval + 1, T - 1) + 1, T - 1);
+    st_c[threadIdx.x] = s_q[nx];
+    st_p[threadIdx.x] = s_tot[t];
+  }
+  if (threadIdx.x == 0) {
+    float m = -__builtin_inff();
+    for (int v = 0; v < V; ++v) m = fmaxf(m, s_tot[s_top[v]]);
+    float sum = 0.f;
+    for (int v = 0; v < V; ++v) sum += expf(s_tot[s_top[v]] - m);
+    if (m + logf(sum) > d.log_stop) atomicOr(flags + s + 1, 1);
+  }
+}
+
+}  // namespace lsim
