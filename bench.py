@@ -13,6 +13,7 @@ Adam fit of the 7 parameters on those 32 events (steps/s, parameter recovery), a
 baseline (the float32 NumPy restatement of the reference algorithm on a bounded sample, on the
 host cores).
 
+``--sdtw-only`` runs only the soft-DTW leg (value and gradient between two hit ADC sequences).
 ``--prob-fee-only`` runs only the probabilistic readout on the 200k-segment event's rows.
 ``--chop-only`` runs only the track-chopping leg (350k track rows -> ~3.5M segment rows).
 ``--batched-bwd-only`` runs only the 32-event forward+backward graph replays (for kernel profiles).
@@ -226,6 +227,46 @@ def prob_fee_leg(params, wfs, steps, cpu=True, cpu_rows=16):
     return leg
 
 
+def sdtw_leg(params, lut, trk, fields, steps, cpu=True):
+    """Soft-DTW loss (value + gradient) between the hit ADC sequences of the 200k-segment event at
+    nominal and at perturbed parameters, inputs resident; events around the whole lsim_sdtw call.
+    CPU baseline: the float64 NumPy restatement on the first 2 000 x 2 000 values."""
+    import numpy as np
+    import torch
+
+    from larndsim_amd import synthetic
+    from larndsim_amd.losses import _sdtw_hip
+    from larndsim_amd.strategies import HipLUTSimulation
+
+    strat = HipLUTSimulation(lut)
+    with torch.no_grad():
+        a = strat.predict(params, trk, fields)["adcs"].reshape(-1)
+        b = strat.predict(synthetic.perturbed_params(params, seed=4), trk, fields)["adcs"].reshape(-1)
+    x, y = a[a > 0].contiguous(), b[b > 0].contiguous()
+    _sdtw_hip(x, y, 1.0, True)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        _sdtw_hip(x, y, 1.0, True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    cells = x.numel() * y.numel()
+    leg = dict(n=x.numel(), m=y.numel(), ms_value_and_grad=ms, cells_per_s=2 * cells / (ms * 1e-3))
+    if cpu:
+        sys.path.insert(0, ROOT)
+        from oracle import sdtw as SD
+
+        xs, ys = x[:2000].cpu().double().numpy(), y[:2000].cpu().double().numpy()
+        t0 = time.perf_counter()
+        SD.soft_dtw_grad(xs, ys, 1.0)
+        dt = time.perf_counter() - t0
+        leg["cpu_baseline"] = dict(value=2 * xs.size * ys.size / dt, unit="cells/s", cores=1, kind="port",
+                                   sample=f"2000 x 2000 values, float64 NumPy by anti-diagonals, {dt:.2f} s")
+    return leg
+
+
 def batched_fwd_bwd(simr, params, dev, steps, fd_baseline=True):
     """32 uneven events (20k-200k segments): forward + backward replayed as one hipGraph, waveform
     rows recomputed (not stored), against the per-event forward-then-finite-difference gradient."""
@@ -313,6 +354,8 @@ def main():
     ap.add_argument("--no-chop", action="store_true", help="skip the track-chopping leg")
     ap.add_argument("--chop-only", action="store_true", help="profiling: only the track-chopping leg")
     ap.add_argument("--no-prob-fee", action="store_true", help="skip the probabilistic readout leg")
+    ap.add_argument("--no-sdtw", action="store_true", help="skip the soft-DTW leg")
+    ap.add_argument("--sdtw-only", action="store_true", help="profiling: only the soft-DTW leg")
     ap.add_argument("--prob-fee-only", action="store_true",
                     help="profiling: only the probabilistic readout on the 200k-segment event's rows")
     ap.add_argument("--batched-bwd-only", action="store_true",
@@ -347,6 +390,10 @@ def main():
         if a.waves:
             S.set_row_waves(a.waves)
         print(json.dumps(batched_fwd_bwd(S.Simulator(params, lut), params, dev, a.steps, fd_baseline=False)))
+        return
+    if a.sdtw_only:
+        tracks, fields = synthetic.workload("c2")
+        print(json.dumps(sdtw_leg(params, lut, torch.from_numpy(tracks).to(dev), fields, a.steps, cpu=not a.no_cpu)))
         return
     if a.prob_fee_only:
         tracks, fields = synthetic.workload("c2")
@@ -469,6 +516,12 @@ def main():
                                          cpu=not a.no_cpu)
         log(f"[bench] prob fee {extra['prob_fee']['rows']} rows: {extra['prob_fee']['ms']:.2f} ms")
         del wst
+
+    # ---- soft-DTW fit loss on the event's hit sequences ----
+    if not a.no_sdtw:
+        log("[bench] soft-DTW")
+        extra["sdtw"] = sdtw_leg(params, lut, trk, fields, max(3, a.steps // 4), cpu=not a.no_cpu)
+        log(f"[bench] sdtw {extra['sdtw']['n']} x {extra['sdtw']['m']}: {extra['sdtw']['ms_value_and_grad']:.2f} ms")
 
     # ---- track chopping (the data step before the path) ----
     if not a.no_chop:

@@ -257,6 +257,19 @@ int lsim_prob_fee(const float *wfs, int64_t n_rows, int32_t n_ticks, int32_t n_p
                   float threshold, float noise_sigma, float t_sampling, float stop_threshold, void *ws,
                   size_t ws_bytes, float *log_prob, float *charge, void *stream);
 
+/* ---- Soft-DTW fit losses ------------------------------------------------------------------------- */
+
+/* Workspace bytes of lsim_sdtw for sequences of n and m values (n * m < 2^36). */
+int lsim_sdtw_query(int64_t n, int64_t m, size_t *ws_bytes);
+
+/* Soft dynamic time warping between x [n] and y [m] (float32, device) with the squared-Euclidean
+ * ground cost, as SoftDTW(gamma).pairwise (src/larndsim/softdtw_jax.py:205-275) computes it for the
+ * sdtw losses (losses_jax.py:309-327): value [1] float32 (device).  When grad_x is not NULL it
+ * receives d value / d x [n] (float64, device).  Deterministic: a tiled anti-diagonal wavefront,
+ * one launch per tile diagonal, fixed-order gradient sums. */
+int lsim_sdtw(const float *x, int64_t n, const float *y, int64_t m, float gamma, void *ws, size_t ws_bytes,
+              float *value, double *grad_x, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include "chop.h"
 #include "loss.h"
 #include "probfee.h"
+#include "sdtw.h"
 
 using namespace lsim;
 
@@ -867,6 +868,56 @@ int lsim_prob_fee(const float* wfs, int64_t n_rows, int32_t n_ticks, int32_t n_p
                        state, flags, log_prob, charge);
     HIP_TRY(hipGetLastError());
   }
+  return LSIM_OK;
+}
+
+// ---- soft-DTW -----------------------------------------------------------------------------------
+
+__global__ void k_sdtw_value(const float* R, int64_t n, int64_t m, float* value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) value[0] = R[(n - 1) * m + (m - 1)];
+}
+
+static size_t sdtw_ws_bytes(int64_t n, int64_t m) {
+  const int64_t nTJ = (m + kDtwTile - 1) / kDtwTile;
+  return (size_t)(2 * n * m + n * nTJ) * sizeof(float);
+}
+
+int lsim_sdtw_query(int64_t n, int64_t m, size_t* ws_bytes) {
+  if (n < 1 || m < 1 || !ws_bytes) return fail(LSIM_E_ARG, "bad arguments");
+  if (n >= ((int64_t)1 << 31) || m >= ((int64_t)1 << 31) || n * m >= ((int64_t)1 << 36))
+    return fail(LSIM_E_ARG, "sequences too long");
+  *ws_bytes = sdtw_ws_bytes(n, m);
+  return LSIM_OK;
+}
+
+int lsim_sdtw(const float* x, int64_t n, const float* y, int64_t m, float gamma, void* ws, size_t ws_bytes,
+              float* value, double* grad_x, void* stream) {
+  size_t need = 0;
+  int rc = lsim_sdtw_query(n, m, &need);
+  if (rc != LSIM_OK) return rc;
+  if (!(gamma > 0.f)) return fail(LSIM_E_ARG, "gamma must be > 0");
+  if (!x || !y || !ws || !value) return fail(LSIM_E_ARG, "NULL argument");
+  if (ws_bytes < need) return fail(LSIM_E_SPACE, "soft-DTW workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  float* R = (float*)ws;
+  float* E = R + n * m;
+  float* part = E + n * m;
+  const int nTI = (int)((n + kDtwTile - 1) / kDtwTile), nTJ = (int)((m + kDtwTile - 1) / kDtwTile);
+  const unsigned diag = (unsigned)(nTI < nTJ ? nTI : nTJ);
+  for (int k = 0; k < nTI + nTJ - 1; ++k) {
+    hipLaunchKernelGGL(k_sdtw_fwd, dim3(diag), dim3(kDtwTile), 0, st, x, n, y, m, gamma, k, R);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_sdtw_value, dim3(1), dim3(64), 0, st, (const float*)R, n, m, value);
+  HIP_TRY(hipGetLastError());
+  if (!grad_x) return LSIM_OK;
+  for (int k = nTI + nTJ - 2; k >= 0; --k) {
+    hipLaunchKernelGGL(k_sdtw_bwd, dim3(diag), dim3(kDtwTile), 0, st, x, n, y, m, gamma, k, (const float*)R, E, part);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_sdtw_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const float*)part, n, nTJ,
+                     grad_x);
+  HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }
 

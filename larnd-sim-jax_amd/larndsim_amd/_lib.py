@@ -20,7 +20,7 @@ EXPORTS = (
     "lsim_forward_query", "lsim_forward_prepare", "lsim_forward_rows", "lsim_forward", "lsim_fee_query",
     "lsim_fee_forward", "lsim_backward_query", "lsim_backward", "lsim_set_row_waves", "lsim_segment_records", "lsim_timing",
     "lsim_timing_read", "lsim_rbf_sums", "lsim_nearest", "lsim_chop_query", "lsim_chop_count", "lsim_chop_fill",
-    "lsim_prob_fee_query", "lsim_prob_fee",
+    "lsim_prob_fee_query", "lsim_prob_fee", "lsim_sdtw_query", "lsim_sdtw",
 )
 
 
@@ -110,6 +110,8 @@ def load(path: str = LIB_PATH):
         "lsim_nearest": [vp, i64, vp, vp, vp, i64, i64, vp, vp, vp],
         "lsim_prob_fee_query": [i64, i32, i32, i32, C.POINTER(sz)],
         "lsim_prob_fee": [vp, i64, i32, i32, i32, i32, C.c_float, C.c_float, C.c_float, C.c_float, vp, sz, vp, vp, vp],
+        "lsim_sdtw_query": [i64, i64, C.POINTER(sz)],
+        "lsim_sdtw": [vp, i64, vp, i64, C.c_float, vp, sz, vp, vp, vp],
         "lsim_chop_query": [i64, C.POINTER(sz)],
         "lsim_chop_count": [C.POINTER(ChopLayout), vp, i64, C.c_double, vp, sz, vp, vp],
         "lsim_chop_fill": [C.POINTER(ChopLayout), vp, i64, C.c_double, vp, sz, i64, i64, vp, vp, i64, i64, vp, vp,

@@ -400,6 +400,94 @@ def nll_loss(params, Q, x, y, z, ticks, hit_prob, event, ref_Q, ref_x, ref_y, re
     return p.to(torch.float64).sum() + t1, dict()
 
 
+# --- soft-DTW losses (losses_jax.py:309-327, softdtw_jax.py:205-275) -------------------------------
+
+def _sdtw_hip(x, y, gamma, want_grad):
+    import ctypes as C
+
+    from . import _lib
+
+    lib = _lib.load()
+    x = x.detach().to(torch.float32).contiguous()
+    y = y.detach().to(torch.float32).contiguous()
+    if not x.is_cuda:
+        raise _lib.LsimError("soft-DTW runs on the GPU (lsim_sdtw)")
+    n, m = x.numel(), y.numel()
+    ws = C.c_size_t()
+    _lib.check(lib.lsim_sdtw_query(n, m, C.byref(ws)))
+    buf = torch.empty(ws.value, dtype=torch.uint8, device=x.device)
+    val = torch.empty(1, dtype=torch.float32, device=x.device)
+    g = torch.empty(n, dtype=torch.float64, device=x.device) if want_grad else None
+    st = C.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+    _lib.check(lib.lsim_sdtw(C.c_void_p(x.data_ptr()), n, C.c_void_p(y.data_ptr()), m, C.c_float(float(gamma)),
+                             C.c_void_p(buf.data_ptr()), buf.numel(), C.c_void_p(val.data_ptr()),
+                             C.c_void_p(g.data_ptr()) if g is not None else None, st))
+    return val[0], g
+
+
+class _SoftDTWFn(torch.autograd.Function):
+    """Soft-DTW value of two 1-D sequences; gradients to either (the second by symmetry:
+    sdtw(x, y) = sdtw(y, x))."""
+
+    @staticmethod
+    def forward(ctx, x, y, gamma):
+        v, gx = _sdtw_hip(x, y, gamma, ctx.needs_input_grad[0])
+        gy = _sdtw_hip(y, x, gamma, True)[1] if ctx.needs_input_grad[1] else None
+        ctx.save_for_backward(*(t for t in (gx, gy) if t is not None))
+        ctx.has = (gx is not None, gy is not None)
+        ctx.dtypes = (x.dtype, y.dtype)
+        return v.to(torch.float64)
+
+    @staticmethod
+    def backward(ctx, g):
+        saved = list(ctx.saved_tensors)
+        gx = (g * saved.pop(0)).to(ctx.dtypes[0]) if ctx.has[0] else None
+        gy = (g * saved.pop(0)).to(ctx.dtypes[1]) if ctx.has[1] else None
+        return gx, gy, None
+
+
+class SoftDTW:
+    """Soft dynamic time warping cost (softdtw_jax.py:205-275) with the squared-Euclidean ground
+    cost, on the GPU (``lsim_sdtw``); ``pairwise(x, y)`` takes two 1-D sequences."""
+
+    def __init__(self, gamma=1.0, ground_cost=None, debiased=False, **_):
+        if ground_cost is not None:
+            raise NotImplementedError("only the squared-Euclidean ground cost is implemented")
+        self.gamma = float(gamma)
+        self.debiased = debiased
+
+    def pairwise(self, x, y):
+        x, y = x.reshape(-1), y.reshape(-1)
+        c = _SoftDTWFn.apply(x, y, self.gamma)
+        if self.debiased:
+            return c - 0.5 * (_SoftDTWFn.apply(x, x.detach(), self.gamma) * 2 - _SoftDTWFn.apply(x.detach(), x.detach(), self.gamma)
+                              + _SoftDTWFn.apply(y, y.detach(), self.gamma) * 2 - _SoftDTWFn.apply(y.detach(), y.detach(), self.gamma))
+        return c
+
+
+def sdtw_loss(adcs, ref, dstw):
+    """Soft-DTW between the positive entries of the flattened arrays, in order (losses_jax.py:309-316)."""
+    a, r = adcs.reshape(-1), ref.reshape(-1)
+    return dstw.pairwise(a[a > 0], r[r > 0]), dict()
+
+
+def sdtw_adc(params, adcs, pixels, ticks, ref, pixels_ref, ticks_ref, dstw):
+    """losses_jax.py:318-319."""
+    return sdtw_loss(adcs, ref, dstw)
+
+
+def sdtw_time(params, adcs, pixels, ticks, ref, pixels_ref, ticks_ref, dstw):
+    """losses_jax.py:321-322."""
+    return sdtw_loss(ticks, ticks_ref, dstw)
+
+
+def sdtw_time_adc(params, adcs, pixels, ticks, ref, pixels_ref, ticks_ref, dstw, alpha=0.5):
+    """losses_jax.py:324-327."""
+    la, _ = sdtw_adc(params, adcs, pixels, ticks, ref, pixels_ref, ticks_ref, dstw)
+    lt, _ = sdtw_time(params, adcs, pixels, ticks, ref, pixels_ref, ticks_ref, dstw)
+    return alpha * la + (1 - alpha) * lt, dict()
+
+
 def loss_functions(adc_norm=10.0, match_z=False):
     """The named fit losses of the reference's driver (optimize/fit_params.py:302-312) that run
     here, as (function, keyword arguments) for ``GenericLossStrategy(fn, **kw)``."""
@@ -407,4 +495,8 @@ def loss_functions(adc_norm=10.0, match_z=False):
         "mse_adc": (mse_adc_blockwise, {}),
         "chamfer_3d": (chamfer_3d, {"adc_norm": adc_norm, "match_z": match_z}),
         "nll": (nll_loss, {"adc_norm": adc_norm, "sigma": 1.0}),
+        # the soft-DTW losses take a SoftDTW object as ``dstw`` (fit_params.py:334-335)
+        "sdtw_adc": (sdtw_adc, {"dstw": SoftDTW(gamma=1.0)}),
+        "sdtw_time": (sdtw_time, {"dstw": SoftDTW(gamma=1.0)}),
+        "sdtw_time_adc": (sdtw_time_adc, {"dstw": SoftDTW(gamma=1.0), "alpha": 0.5}),
     }
