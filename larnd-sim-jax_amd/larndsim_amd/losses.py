@@ -44,6 +44,15 @@ def mse_loss(adcs, pIDs, adcs_ref, pIDs_ref):
     return (sig ** 2).sum(), dict()
 
 
+def mse_time(params, adcs, pixels, ticks, ref, pixels_ref, ticks_ref):
+    """Per-pixel squared difference of the hit ticks, ticks of empty ADC slots set to 0
+    (losses_jax.py:82-87).  (mse_time_adc, :89-92, calls mse_adc with the wrong number of
+    arguments in the reference and cannot run there; it is not provided.)"""
+    t = torch.where(adcs > 0, ticks, torch.zeros_like(ticks))
+    t_ref = torch.where(ref > 0, ticks_ref, torch.zeros_like(ticks_ref))
+    return mse_loss(t, pixels, t_ref, pixels_ref)
+
+
 def pixel_charge_loss(params, Q, hit_pixels, ref_Q, ref_hit_pixels):
     """Per-pixel charge loss: mse_loss on the summed hit charge of each pixel."""
     return mse_loss(Q[:, None], hit_pixels, ref_Q[:, None], ref_hit_pixels)

@@ -103,3 +103,17 @@ def test_gradients_match_central_differences(which):
             fq = (f(y, q + e) - f(y, q - e)) / (2 * h)
         assert float(gy[k]) == pytest.approx(float(fy), rel=1e-5, abs=1e-8)
         assert float(gq[k]) == pytest.approx(float(fq), rel=1e-5, abs=1e-8)
+
+
+def test_mse_time_closed_form():
+    """losses_jax.py:82-87: ticks of empty ADC slots count as 0, then the per-pixel squared
+    difference over the union of pixels."""
+    from larndsim_amd.losses import mse_time
+
+    adcs = torch.tensor([[80.0, 0.0], [90.0, 95.0]])
+    ticks = torch.tensor([[10.0, 1998.0], [20.0, 30.0]])
+    ref = torch.tensor([[85.0, 70.0], [0.0, 0.0]])
+    ticks_ref = torch.tensor([[12.0, 40.0], [1998.0, 1998.0]])
+    L, _ = mse_time(None, adcs, torch.tensor([5, 9]), ticks, ref, torch.tensor([5, 7]), ticks_ref)
+    # pixel 5: (10 - 12)^2 + (0 - 40)^2; pixel 7: 0 (masked); pixel 9: 20^2 + 30^2
+    assert float(L) == pytest.approx(4 + 1600 + 400 + 900)
