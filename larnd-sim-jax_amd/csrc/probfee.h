@@ -53,14 +53,22 @@ __device__ __forceinline__ float pf_log_ndtr(float x) {
   return x < 0.f ? -u2 + logf(0.5f * ex) : log1pf(-0.5f * expf(-u2) * ex);
 }
 
-__device__ __forceinline__ float pf_softplus(float x) { return fmaxf(x, 0.f) + log1pf(expf(-fabsf(x))); }
+// softplus(x) = max(x, 0) + log1p(exp(-|x|)); beyond |x| = 30 the correction is below float32
+// resolution (x itself, or exp(x) = log1p(exp(x)) to 1e-13), so the transcendental calls are skipped.
+__device__ __forceinline__ float pf_softplus(float x) {
+  if (x > 30.f) return x;
+  if (x < -30.f) return expf(x);
+  return fmaxf(x, 0.f) + log1pf(expf(-fabsf(x)));
+}
 
 // lo + softplus((x - lo) s) / s (fee_jax.py:13-20)
 __device__ __forceinline__ float pf_soft_max(float x, float lo, float s) { return lo + pf_softplus((x - lo) * s) / s; }
 
 // sigmoid(c s) t + (1 - sigmoid(c s)) f (fee_jax.py:22-30)
 __device__ __forceinline__ float pf_soft_where(float c, float t, float f, float s) {
-  const float w = 1.f / (1.f + expf(-(c * s)));
+  const float x = c * s;
+  // sigmoid(x): 1 in float32 above 30 (1 + exp(-30) rounds to 1); exp(x) below -30 (to 1e-13)
+  const float w = x > 30.f ? 1.f : (x < -30.f ? expf(x) : 1.f / (1.f + expf(-x)));
   return w * t + (1.f - w) * f;
 }
 
@@ -204,8 +212,12 @@ __global__ __launch_bounds__(kPfThreads) void k_pf_step(PfDev d, int s, const fl
                                                        float* __restrict__ out_logp, float* __restrict__ out_q) {
   __shared__ float s_q[kPfMaxTicks];
   __shared__ float s_cm[kPfMaxTicks];
-  __shared__ float s_tot[kPfMaxTicks];
-  __shared__ float s_sel[kPfMaxTicks];
+  // per-path tables log Phi(z(q_sum - c)) and log Phi(z(CM - c)); after the path loop the same
+  // memory holds the per-tick totals and selection scores
+  __shared__ float s_lq[kPfMaxTicks];
+  __shared__ float s_lc[kPfMaxTicks];
+  float* const s_tot = s_lq;
+  float* const s_sel = s_lc;
   __shared__ float s_c[kPfMaxPaths], s_p[kPfMaxPaths];
   __shared__ float s_bv[kPfThreads / kWave];
   __shared__ int s_bi[kPfThreads / kWave];
@@ -270,30 +282,32 @@ __global__ __launch_bounds__(kPfThreads) void k_pf_step(PfDev d, int s, const fl
   float espk[kPfChunk];
 #pragma unroll
   for (int k = 0; k < kPfChunk; ++k) espk[k] = t0 + k < n ? esp[t0 + k] : 0.f;
-  // one pass over the (merged) paths with running logsumexp accumulators (max and shifted sum)
+  // one pass over the (merged) paths with running logsumexp accumulators (max and shifted sum);
+  // each path's log Phi values are tabulated once per tick (each is used by two ticks)
   for (int v = 0; v < NV; ++v) {
     const float c = s_c[v], pv = s_p[v];
-    float b1 = (s_cm[min(t0, T - 1)] - c - d.theta) * d.zs;
-    float lb1 = pf_log_ndtr(b1);
+    for (int t = threadIdx.x; t < T; t += kPfThreads) {
+      s_lq[t] = pf_log_ndtr((s_q[t] - c - d.theta) * d.zs);
+      s_lc[t] = pf_log_ndtr((s_cm[t] - c - d.theta) * d.zs);
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPfChunk; ++k) {
       const int t = t0 + k;
       if (t < n) {
-        const float a1 = (s_cm[t + 1] - c - d.theta) * d.zs;
-        const float la1 = pf_log_ndtr(a1);
-        const float lg = pf_log_diff(a1, b1, la1, lb1);
+        const float a1 = (s_cm[t + 1] - c - d.theta) * d.zs, b1 = (s_cm[t] - c - d.theta) * d.zs;
+        const float lg = pf_log_diff(a1, b1, s_lc[t + 1], s_lc[t]);
         const int sh = min(t + d.interval + 1, T - 1);
         const float a2 = (s_q[sh] - c - d.theta) * d.zs, b2 = (s_q[t] - c - d.theta) * d.zs;
-        float le = pf_log_diff(a2, b2, pf_log_ndtr(a2), pf_log_ndtr(b2));
+        float le = pf_log_diff(a2, b2, s_lq[sh], s_lq[t]);
         le = fminf(le, lg);
         le = pf_soft_max(le, -1000.f, 1.f);
         le = pf_soft_where(espk[k] - d.theta, le, -1000.f, 10.f);
         pf_lse_add(hmax[k], hsum[k], le + pv);
         pf_lse_add(gmax[k], gsum[k], lg + pv);
-        b1 = a1;
-        lb1 = la1;
       }
     }
+    __syncthreads();
   }
 #pragma unroll
   for (int k = 0; k < kPfChunk; ++k) {
