@@ -235,3 +235,31 @@ def test_mmd_closed_form():
     b = torch.tensor([[1.0, 2.0, 2.0]], dtype=torch.float64)
     one = torch.ones(1, dtype=torch.float64)
     assert float(losses.mmd(a, b, one, one, 1.5)) == pytest.approx(2 - 2 * np.exp(-9 / (2 * 1.5 ** 2)), rel=1e-12)
+
+
+def test_new_entry_points_check_arguments(lib):
+    """The readout, soft-DTW, nearest-neighbour and chop entry points reject bad arguments on the
+    host (no device access): sizes, limits and NULL pointers."""
+    import ctypes as C
+
+    sz = C.c_size_t()
+    assert lib.lsim_prob_fee_query(10, 2000, 20, 10, C.byref(sz)) == 0 and sz.value > 10 * 4 * 2000 * 4
+    assert lib.lsim_prob_fee_query(10, 1, 20, 10, C.byref(sz)) < 0
+    # n_ticks beyond 2048, more paths than 32, noise 0
+    assert lib.lsim_prob_fee(None, 1, 4096, 20, 10, 18, C.c_float(7e3), C.c_float(900.0), C.c_float(0.1),
+                             C.c_float(1e-9), None, 0, None, None, None) < 0
+    assert lib.lsim_prob_fee(None, 1, 2000, 33, 10, 18, C.c_float(7e3), C.c_float(900.0), C.c_float(0.1),
+                             C.c_float(1e-9), None, 0, None, None, None) < 0
+    assert lib.lsim_prob_fee(None, 1, 2000, 20, 10, 18, C.c_float(7e3), C.c_float(0.0), C.c_float(0.1),
+                             C.c_float(1e-9), None, 1 << 30, None, None, None) < 0
+    assert b"noise_sigma" in lib.lsim_last_error()
+    assert lib.lsim_sdtw_query(100, 200, C.byref(sz)) == 0 and sz.value >= 2 * 100 * 200 * 4
+    assert lib.lsim_sdtw_query(0, 5, C.byref(sz)) < 0
+    assert lib.lsim_sdtw_query(1 << 20, 1 << 20, C.byref(sz)) < 0  # n * m beyond 2^36
+    assert lib.lsim_sdtw(None, 10, None, 10, C.c_float(1.0), None, 0, None, None, None) < 0
+    assert lib.lsim_sdtw(C.c_void_p(8), 10, C.c_void_p(8), 10, C.c_float(0.0), C.c_void_p(8), 1 << 20,
+                         C.c_void_p(8), None, None) < 0
+    assert b"gamma" in lib.lsim_last_error()
+    assert lib.lsim_nearest(None, 5, None, None, None, 5, 6, None, None, None) < 0  # n_origin > nx
+    assert lib.lsim_nearest(None, 0, None, None, None, 0, 0, None, None, None) == 0  # nothing to do
+    assert lib.lsim_chop_query(0, C.byref(sz)) == 0
