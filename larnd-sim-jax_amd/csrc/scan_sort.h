@@ -94,12 +94,46 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const T* __restrict
 
 inline int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
+// Small inputs (up to 16 values per thread of one 1024-thread workgroup): the whole scan in one
+// launch instead of three (the radix sort's digit histograms and the per-row counts are this
+// size, and each launch costs a few microseconds on its own).
+constexpr int kScanSmallThreads = 1024;
+constexpr int kScanSmallItems = 16;
+constexpr int64_t kScanSmallMax = (int64_t)kScanSmallThreads * kScanSmallItems;
+
+template <typename T, typename TT>
+__global__ __launch_bounds__(kScanSmallThreads) void k_scan_small(const T* in, T* out, int64_t n,
+                                                                 TT* __restrict__ total) {  // in may equal out
+  __shared__ T sh[kScanSmallThreads / kWave];
+  const int per = (int)((n + kScanSmallThreads - 1) / kScanSmallThreads);  // <= kScanSmallItems
+  const int64_t base = (int64_t)threadIdx.x * per;
+  T v[kScanSmallItems];
+  T s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanSmallItems; ++k) {
+    v[k] = (k < per && base + k < n) ? in[base + k] : T(0);
+    s += v[k];
+  }
+  T tot;
+  T ex = block_exclusive_scan<T>(s, sh, tot);
+#pragma unroll
+  for (int k = 0; k < kScanSmallItems; ++k) {
+    if (k < per && base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0 && total) *total = (TT)tot;
+}
+
 // Exclusive scan of n values (in may equal out).  tmp holds scan_tiles(n) values of T.
 template <typename T, typename TT>
 inline hipError_t exclusive_scan(const T* in, T* out, int64_t n, T* tmp, TT* total, hipStream_t st) {
   const int64_t nt = scan_tiles(n);
   if (nt == 0) {
     return hipSuccess;
+  }
+  if (n <= kScanSmallMax) {
+    hipLaunchKernelGGL((k_scan_small<T, TT>), dim3(1), dim3(kScanSmallThreads), 0, st, in, out, n, total);
+    return hipGetLastError();
   }
   hipLaunchKernelGGL((k_scan_reduce<T>), dim3((unsigned)nt), dim3(kScanThreads), 0, st, in, n, tmp);
   hipLaunchKernelGGL((k_scan_tiles<T, TT>), dim3(1), dim3(kScanThreads), 0, st, tmp, nt, total);
