@@ -362,8 +362,8 @@ def main():
                     help="profiling: only the 32-event batched forward+backward graph replays (no baseline)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events shard across GPUs "
-              "with no exchange in the forward; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events would shard across "
+              "GPUs with no exchange in the forward and one 8-value all-reduce per gradient step)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
