@@ -48,6 +48,47 @@ class HipLUTSimulation(SimulationStrategy):
                     unique_pixels=st.out["pixel_ids"][: st.max_rows], wfs=st.out["wfs"][: st.max_rows])
 
 
+def pad_to_closest_multiple(x, dims_to_pad=None, multiple=128, pad_value=0, pad_front=False):
+    """Pad the given dimensions of x up to the next multiple (strategies.py:41-77), at the front or
+    the back."""
+    x = torch.as_tensor(x)
+    dims = range(x.dim()) if dims_to_pad is None else dims_to_pad
+    shape = list(x.shape)
+    for d in dims:
+        shape[d] = ((x.shape[d] + multiple - 1) // multiple) * multiple
+    out = torch.full(shape, pad_value, dtype=x.dtype, device=x.device)
+    sl = tuple(slice(shape[i] - n, None) if pad_front else slice(0, n) for i, n in enumerate(x.shape))
+    out[sl] = x
+    return out
+
+
+class HipLUTProbabilisticSimulation(SimulationStrategy):
+    """LUT waveforms and the probabilistic readout on the GPU (strategies.py:107-134): the raw
+    (pixel, hit index, tick) distributions for the probabilistic losses.  The readout's adjoint is
+    not built, so parameters that require gradients are refused."""
+
+    def __init__(self, response):
+        self.response = response
+
+    def predict(self, params, tracks, fields, rngkey=None):
+        from .detsim import id2pixel
+        from .fee import simulate_probabilistic
+        from .sim import simulate_wfs
+
+        if needs_grad(params):
+            raise NotImplementedError("gradients through the probabilistic readout are not available")
+        # the waveforms do not depend on the readout noise, which only the readout uses
+        wfs, unique_pixels = simulate_wfs(params.replace(RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0),
+                                          self.response, tracks, fields)
+        unique_pixels = pad_to_closest_multiple(unique_pixels, multiple=128, pad_value=-1, pad_front=True)
+        wfs = pad_to_closest_multiple(wfs, dims_to_pad=(0,), multiple=128, pad_value=0.0, pad_front=True)
+        adcs_distrib, pixel_x, pixel_y, ticks_prob, event = simulate_probabilistic(params, wfs, unique_pixels)
+        _, _, pixel_plane, _ = id2pixel(params, unique_pixels)
+        return dict(adcs_distrib=adcs_distrib, hit_prob=ticks_prob, pixel_x=pixel_x, pixel_y=pixel_y,
+                    pixel_plane=pixel_plane, event=event, unique_pixels=unique_pixels, hit_pixels=unique_pixels,
+                    wfs=wfs)
+
+
 def _last_others(t):
     return t.grad_fn.others if t.grad_fn is not None else {}
 

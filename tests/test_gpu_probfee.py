@@ -95,3 +95,26 @@ def test_probabilistic_outputs_and_stop(params4):
     xy = detsim.get_pixel_coordinates(params, torch.tensor([3, 10, 20]), torch.tensor([4, 5, 6]),
                                       torch.tensor([0, 1, 0]))
     assert torch.allclose(px.cpu().double(), xy[:, 0].double()) and torch.allclose(py.cpu().double(), xy[:, 1].double())
+
+
+def test_probabilistic_strategy(params4, lut, c1):
+    """HipLUTProbabilisticSimulation (strategies.py:107-134): rows padded at the front to a multiple
+    of 128 with pixel id -1, the readout of those rows, digitised expected charges."""
+    from larndsim_amd.fee import digitize, get_adc_values_average_noise_vmap
+    from larndsim_amd.strategies import HipLUTProbabilisticSimulation
+
+    tracks, fields = c1
+    p = params4.replace(RESET_NOISE_CHARGE=900.0)
+    out = HipLUTProbabilisticSimulation(lut).predict(p, tracks, fields)
+    P = out["wfs"].shape[0]
+    print("probabilistic strategy rows", P, flush=True)
+    assert P % 128 == 0 and tuple(out["hit_prob"].shape) == (P, 10, 1999)
+    ids = out["unique_pixels"]
+    npad = int((ids == -1).sum())
+    assert npad > 0 and bool((ids[:npad] == -1).all()) and bool((out["wfs"][:npad] == 0).all())
+    lp, q = get_adc_values_average_noise_vmap(p, out["wfs"])
+    assert torch.equal(lp, out["hit_prob"])
+    assert torch.equal(digitize(p, q), out["adcs_distrib"])
+    with pytest.raises(NotImplementedError):
+        HipLUTProbabilisticSimulation(lut).predict(
+            p.replace(Ab=torch.tensor(float(p.Ab), dtype=torch.float64, requires_grad=True)), tracks, fields)
