@@ -259,7 +259,7 @@ int lsim_prob_fee(const float *wfs, int64_t n_rows, int32_t n_ticks, int32_t n_p
 
 /* ---- Soft-DTW fit losses ------------------------------------------------------------------------- */
 
-/* Workspace bytes of lsim_sdtw for sequences of n and m values (n * m < 2^36). */
+/* Workspace bytes of lsim_sdtw for sequences of n and m values (n * m <= 2^34; 8 bytes per cell). */
 int lsim_sdtw_query(int64_t n, int64_t m, size_t *ws_bytes);
 
 /* Soft dynamic time warping between x [n] and y [m] (float32, device) with the squared-Euclidean

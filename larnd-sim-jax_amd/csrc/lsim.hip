@@ -884,8 +884,9 @@ static size_t sdtw_ws_bytes(int64_t n, int64_t m) {
 
 int lsim_sdtw_query(int64_t n, int64_t m, size_t* ws_bytes) {
   if (n < 1 || m < 1 || !ws_bytes) return fail(LSIM_E_ARG, "bad arguments");
-  if (n >= ((int64_t)1 << 31) || m >= ((int64_t)1 << 31) || n * m >= ((int64_t)1 << 36))
-    return fail(LSIM_E_ARG, "sequences too long");
+  // R and E take 8 bytes per cell: 2^34 cells = 137 GB, half of one GPU's HBM
+  if (n >= ((int64_t)1 << 31) || m >= ((int64_t)1 << 31) || n * m > ((int64_t)1 << 34))
+    return fail(LSIM_E_ARG, "sequences too long (n * m above 2^34 cells)");
   *ws_bytes = sdtw_ws_bytes(n, m);
   return LSIM_OK;
 }

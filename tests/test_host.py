@@ -255,7 +255,7 @@ def test_new_entry_points_check_arguments(lib):
     assert b"noise_sigma" in lib.lsim_last_error()
     assert lib.lsim_sdtw_query(100, 200, C.byref(sz)) == 0 and sz.value >= 2 * 100 * 200 * 4
     assert lib.lsim_sdtw_query(0, 5, C.byref(sz)) < 0
-    assert lib.lsim_sdtw_query(1 << 20, 1 << 20, C.byref(sz)) < 0  # n * m beyond 2^36
+    assert lib.lsim_sdtw_query(1 << 20, 1 << 15, C.byref(sz)) < 0  # n * m beyond 2^34
     assert lib.lsim_sdtw(None, 10, None, 10, C.c_float(1.0), None, 0, None, None, None) < 0
     assert lib.lsim_sdtw(C.c_void_p(8), 10, C.c_void_p(8), 10, C.c_float(0.0), C.c_void_p(8), 1 << 20,
                          C.c_void_p(8), None, None) < 0
