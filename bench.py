@@ -467,6 +467,22 @@ def main():
         extra["fd_gradient_baseline"] = dict(forwards=len(fd_params), ms_per_gradient=dtfd * 1e3,
                                              segments_per_s=N / dtfd, adjoint_speedup=dtfd / dtb)
         log(f"[bench] fwd+bwd {dtb * 1e3:.3f} ms; FD gradient {dtfd * 1e3:.3f} ms (x{dtfd / dtb:.1f})")
+        # the same with a dense cotangent on every waveform tick as well (lsim_backward_wfs)
+        st_w = simr.forward(params, trk, fields, n_events=1, want_tape=True)
+        d_w = torch.from_numpy(rng.normal(size=(st_w.max_rows, simr.nout)).astype(np.float32)).to(dev)
+        d_a = d_adc[: st_w.max_rows * params.MAX_ADC_VALUES]
+        simr.backward(st_w, d_a, None, d_w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            st_w = simr.forward(params, trk, fields, n_events=1, want_tape=True, max_rows=P)
+            simr.backward(st_w, d_a, None, d_w)
+        torch.cuda.synchronize()
+        dtw = (time.perf_counter() - t0) / a.steps
+        extra["fwd_bwd_waveform_cotangent"] = dict(ms_per_step=dtw * 1e3, segments_per_s=N / dtw,
+                                                   fd_speedup=dtfd / dtw)
+        log(f"[bench] fwd+bwd with a waveform cotangent {dtw * 1e3:.3f} ms")
+        del st_w, d_w
 
     # ---- batched forward: 128 uneven events, one graph replay ----
     if not a.no_batched:

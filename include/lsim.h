@@ -182,6 +182,15 @@ int lsim_backward(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg,
                   size_t ws_bytes, int64_t max_rows, const lsim_fwd_out *fwd, const float *d_adc, const float *d_z,
                   void *bwd_ws, size_t bwd_ws_bytes, double *d_params, void *stream);
 
+/* As lsim_backward, plus a cotangent on the waveforms themselves: the gradient of
+ * sum(d_adc * adc) + sum(d_z * hit_z) + sum(d_wfs * wfs), d_wfs [max_rows][nout] float32 (device),
+ * the rows and ticks of the forward's wfs output (the reference differentiates simulate_wfs's
+ * waveforms as well, sim_jax.py:689-736).  The row cotangent becomes a step function with one
+ * piece per tick, so this costs more than lsim_backward. */
+int lsim_backward_wfs(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg, int32_t n_events, void *ws,
+                      size_t ws_bytes, int64_t max_rows, const lsim_fwd_out *fwd, const float *d_adc, const float *d_z,
+                      const float *d_wfs, void *bwd_ws, size_t bwd_ws_bytes, double *d_params, void *stream);
+
 /* Fit losses: event-blocked Gaussian-kernel sums of the hit-cloud MMD in mse_adc
  * (src/larndsim/losses_jax.py:13-39 rbf_kernel / mmd, :58-80 mse_adc) and of the intensity of
  * nll_loss (:329-361), without forming the kernel matrices:

@@ -203,7 +203,7 @@ struct RowG {
   const int* x;
   const float* c;
   const float* dc;
-  const uint8_t* qlut;
+  const uint16_t* qlut;
   int np;
   __device__ __forceinline__ double at(int o) const {  // G(o), 0 <= o < nout
     const int q = qlut[o];
@@ -433,21 +433,26 @@ constexpr bool kAdjBinItems = LSIM_ADJ_BIN_ITEMS != 0;  // one work item per dif
 // partial gradients.  Two passes (separate launches, separate partials): NEAR = false takes the
 // neighbour-only cells (|a| or |b| > 1; little register state, high occupancy), NEAR = true the
 // 3x3 cells with the diffusion bins.  Rows without gradient (no valid hit) exit at once.
-template <int NB, int WAVES, bool NEAR>
+// DENSE: the row's cotangent is a full waveform cotangent gw[row][o] (plus the self-trigger's step
+// function): one breakpoint per output tick, the same telescoped window sums with up to L
+// breakpoints per window.
+template <int NB, int WAVES, bool NEAR, bool DENSE>
 __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_rows_adj(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const PairRec* __restrict__ prs,
                                                             const AdjRec* __restrict__ adj, const int2* __restrict__ rng,
                                                             const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
                                                             const Pieces* __restrict__ pieces,
+                                                            const float* __restrict__ gw,
                                                             double* __restrict__ partial) {
   constexpr int NT = WAVES * kWave;
+  constexpr int kNP = DENSE ? kMaxOut : 2 * kMaxH;  // breakpoints held in LDS
   __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
   __shared__ int ncls[3];
   __shared__ int cell_scratch[3 * kMaxCells];
-  __shared__ int spx[2 * kMaxH];
-  __shared__ float spc[2 * kMaxH], sdc[2 * kMaxH];
-  __shared__ uint8_t qlut[kMaxOut];
+  __shared__ int spx[kNP];
+  __shared__ float spc[kNP], sdc[kNP];
+  __shared__ uint16_t qlut[kMaxOut];
   __shared__ double red[kNGrad][NT / kWave];
   const int t = threadIdx.x;
   const int P0 = *n_rows_dev;
@@ -461,22 +466,40 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
   const int q8 = P / kXcd, rem = P % kXcd;
   const int r = xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
   const Pieces* pr = pieces + r;
-  const int np = pr->n;
-  if (np == 0) {
-    if (t < kNGrad) partial[(int64_t)r * kNGrad + t] = 0.0;
-    return;
-  }
-  if (t < np) {
-    spx[t] = pr->x[t];
-    spc[t] = pr->c[t];
-    sdc[t] = pr->c[t] - (t + 1 < np ? pr->c[t + 1] : 0.f);
-  }
-  __syncthreads();
-  // qlut[o] = number of breakpoints below o = first q with x[q] >= o (np from nout on: G = 0 there)
-  for (int o = t; o < kMaxOut; o += NT) {
-    int q = 0;
-    for (int k = 0; k < np; ++k) q += spx[k] < o ? 1 : 0;
-    qlut[o] = (uint8_t)q;
+  int np;
+  if (DENSE) {
+    // G(o) = gw[r][o] + the self-trigger's step function at o, one breakpoint per output tick
+    np = d.nout;
+    const float* gr = gw + (int64_t)r * d.nout;
+    const int nf = pr->n;
+    for (int o = t; o < np; o += NT) {
+      float cf = 0.f;
+      for (int k = nf - 1; k >= 0; --k)
+        if (pr->x[k] >= o) cf = pr->c[k];
+      spx[o] = o;
+      spc[o] = gr[o] + cf;
+    }
+    __syncthreads();
+    for (int o = t; o < np; o += NT) sdc[o] = spc[o] - (o + 1 < np ? spc[o + 1] : 0.f);
+    for (int o = t; o < kMaxOut; o += NT) qlut[o] = (uint16_t)(o < np ? o : np);
+  } else {
+    np = pr->n;
+    if (np == 0) {
+      if (t < kNGrad) partial[(int64_t)r * kNGrad + t] = 0.0;
+      return;
+    }
+    if (t < np) {
+      spx[t] = pr->x[t];
+      spc[t] = pr->c[t];
+      sdc[t] = pr->c[t] - (t + 1 < np ? pr->c[t + 1] : 0.f);
+    }
+    __syncthreads();
+    // qlut[o] = number of breakpoints below o = first q with x[q] >= o (np from nout on: G = 0 there)
+    for (int o = t; o < kMaxOut; o += NT) {
+      int q = 0;
+      for (int k = 0; k < np; ++k) q += spx[k] < o ? 1 : 0;
+      qlut[o] = (uint16_t)q;
+    }
   }
   __syncthreads();  // qlut complete
   // this pass's cells in (a, b) order, each with its live pairs only: window starts ascend within a

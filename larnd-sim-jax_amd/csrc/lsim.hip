@@ -637,9 +637,9 @@ int lsim_backward_query(lsim_ctx* c, int64_t n_seg, int64_t max_rows, size_t* by
   return LSIM_OK;
 }
 
-int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws, size_t ws_bytes,
-                  int64_t max_rows, const lsim_fwd_out* fwd, const float* d_adc, const float* d_z, void* bwd_ws,
-                  size_t bwd_ws_bytes, double* d_params, void* stream) {
+static int backward_impl(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws,
+                         size_t ws_bytes, int64_t max_rows, const lsim_fwd_out* fwd, const float* d_adc, const float* d_z,
+                         const float* d_wfs, void* bwd_ws, size_t bwd_ws_bytes, double* d_params, void* stream) {
   if (!c || !p || !seg || !ws || !fwd || !fwd->tape || !d_adc || !bwd_ws || !d_params)
     return fail(LSIM_E_ARG, "NULL argument (the forward tape is required)");
   if (!c->has_lut) return fail(LSIM_E_STATE, "lsim_lut_set has not been called");
@@ -673,17 +673,25 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
       hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, in, keys, vals, n, adj);
     const dim3 grid((unsigned)max_rows), block(kAdjWaves * kWave);
     double* pnear = partial + max_rows * kNGrad;
-#define LSIM_ADJ_LAUNCH(NBV)                                                                                        \
-  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,       \
-                     w.row_cell, w.n_rows, max_rows, pieces, partial);                                              \
-  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,        \
-                     w.row_cell, w.n_rows, max_rows, pieces, pnear)
-    if (d.nb == 5) {
-      LSIM_ADJ_LAUNCH(5);
+#define LSIM_ADJ_LAUNCH(NBV, DN)                                                                                    \
+  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false, DN>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,   \
+                     w.row_cell, w.n_rows, max_rows, pieces, d_wfs, partial);                                       \
+  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true, DN>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,    \
+                     w.row_cell, w.n_rows, max_rows, pieces, d_wfs, pnear)
+    if (d_wfs) {
+      if (d.nb == 5) {
+        LSIM_ADJ_LAUNCH(5, true);
+      } else if (d.nb == 3) {
+        LSIM_ADJ_LAUNCH(3, true);
+      } else {
+        LSIM_ADJ_LAUNCH(1, true);
+      }
+    } else if (d.nb == 5) {
+      LSIM_ADJ_LAUNCH(5, false);
     } else if (d.nb == 3) {
-      LSIM_ADJ_LAUNCH(3);
+      LSIM_ADJ_LAUNCH(3, false);
     } else {
-      LSIM_ADJ_LAUNCH(1);
+      LSIM_ADJ_LAUNCH(1, false);
     }
 #undef LSIM_ADJ_LAUNCH
   }
@@ -691,6 +699,21 @@ int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, i
                      max_rows, d_params);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
+}
+
+int lsim_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws, size_t ws_bytes,
+                  int64_t max_rows, const lsim_fwd_out* fwd, const float* d_adc, const float* d_z, void* bwd_ws,
+                  size_t bwd_ws_bytes, double* d_params, void* stream) {
+  return backward_impl(c, p, seg, n_events, ws, ws_bytes, max_rows, fwd, d_adc, d_z, nullptr, bwd_ws, bwd_ws_bytes,
+                       d_params, stream);
+}
+
+int lsim_backward_wfs(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws,
+                      size_t ws_bytes, int64_t max_rows, const lsim_fwd_out* fwd, const float* d_adc, const float* d_z,
+                      const float* d_wfs, void* bwd_ws, size_t bwd_ws_bytes, double* d_params, void* stream) {
+  if (!d_wfs) return fail(LSIM_E_ARG, "NULL d_wfs");
+  return backward_impl(c, p, seg, n_events, ws, ws_bytes, max_rows, fwd, d_adc, d_z, d_wfs, bwd_ws, bwd_ws_bytes,
+                       d_params, stream);
 }
 
 int lsim_rbf_sums(const float* x, int64_t nx, const int32_t* lo, const int32_t* hi, const float* y, const float* c,

@@ -33,15 +33,12 @@ class LUTSimFunction(torch.autograd.Function):
         others["wfs"] = st.out["wfs"][: st.max_rows]
         ctx.others = others
         wfs = st.out["wfs"][: st.max_rows]  # a view object of its own (not the one kept above)
-        ctx.mark_non_differentiable(wfs)
         return h["adcs"], h["pixel_z"], wfs
 
     @staticmethod
     def backward(ctx, g_adc, g_z, g_wfs):
-        if g_wfs is not None and bool(torch.any(g_wfs != 0)):
-            raise NotImplementedError("gradients through the raw waveforms are not implemented; "
-                                      "differentiate the hit outputs (adcs, pixel_z)")
         st, H = ctx.state, ctx.sim.H
+        d_wfs = g_wfs if (g_wfs is not None and bool(torch.any(g_wfs != 0))) else None
         n = max(st.max_rows, 1) * H
         dev = ctx.sim.device
         d_adc = torch.zeros(n, dtype=torch.float32, device=dev)
@@ -51,7 +48,7 @@ class LUTSimFunction(torch.autograd.Function):
         if g_z is not None:
             d_z = torch.zeros(n, dtype=torch.float32, device=dev)
             d_z.index_copy_(0, ctx.hit_index, g_z.to(torch.float32))
-        grad = ctx.sim.backward(st, d_adc, d_z).to(ctx.theta_device)
+        grad = ctx.sim.backward(st, d_adc, d_z, d_wfs).to(ctx.theta_device)
         ctx.state = None  # the workspace and tape are not needed again (no double backward)
         return grad, None, None, None, None, None
 

@@ -265,8 +265,9 @@ class Simulator:
                                              ws.numel(), max_rows, C.byref(oc), st))
         return ForwardState(seg.n, n_events, t, seg, ws, int(max_rows), out, oc, params)
 
-    def backward(self, state: ForwardState, d_adc, d_z=None):
-        """d loss / d (Ab, kb, eField, lifetime, vdrift, long_diff, tran_diff) as float64 (7,)."""
+    def backward(self, state: ForwardState, d_adc, d_z=None, d_wfs=None):
+        """d loss / d (Ab, kb, eField, lifetime, vdrift, long_diff, tran_diff) as float64 (7,), from
+        cotangents on the ADC grid, the hit z grid and (optionally) the waveforms (max_rows, nout)."""
         if state.out.get("tape") is None:
             raise _lib.LsimError("forward was run without want_tape=True")
         sz = C.c_size_t()
@@ -279,6 +280,15 @@ class Simulator:
             dz_p = C.c_void_p(d_z.data_ptr())
         grad = torch.zeros(7, dtype=torch.float64, device=self.device)
         pc = params_struct(state.params)
+        if d_wfs is not None:
+            dw = torch.zeros((state.max_rows, self.nout), dtype=torch.float32, device=self.device)
+            dw[: d_wfs.shape[0]] = d_wfs.to(device=self.device, dtype=torch.float32)
+            _lib.check(self.lib.lsim_backward_wfs(self._ctx, C.byref(pc), C.byref(state.seg), state.n_events,
+                                                  C.c_void_p(state.ws.data_ptr()), state.ws.numel(), state.max_rows,
+                                                  C.byref(state.out_c), C.c_void_p(d_adc.data_ptr()), dz_p,
+                                                  C.c_void_p(dw.data_ptr()), C.c_void_p(bws.data_ptr()), bws.numel(),
+                                                  C.c_void_p(grad.data_ptr()), self._stream()))
+            return grad
         _lib.check(self.lib.lsim_backward(self._ctx, C.byref(pc), C.byref(state.seg), state.n_events,
                                           C.c_void_p(state.ws.data_ptr()), state.ws.numel(), state.max_rows,
                                           C.byref(state.out_c), C.c_void_p(d_adc.data_ptr()), dz_p,

@@ -145,3 +145,56 @@ def test_autograd_releases_workspace(params4, lut, c1):
         assert used[-1] <= used[1]
     finally:
         gc.enable()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_backward_with_waveform_cotangent(sim, params4, c1, frozen, seed):
+    """lsim_backward_wfs: cotangents on the waveforms as well as on the ADC and z grids (the row
+    cotangent becomes one breakpoint per tick) against float64 autograd of the same chain."""
+    tr, f = c1
+    print(f"[gpu] backward with waveform cotangent c1 seed {seed}", flush=True)
+    st = sim.forward(params4, tr, f, want_tape=True)
+    vr = frozen.valid_rows
+    P = st.max_rows
+    assert P == int(vr.sum())
+    w_adc, w_z = _cotangents(frozen, seed)
+    rng = np.random.default_rng(100 + seed)
+    w_wfs = rng.normal(size=(P, sim.nout))
+    th = frozen.theta0().requires_grad_(True)
+    adc, z, wfs = frozen.forward(th)
+    vrt = torch.as_tensor(vr)
+    L = (torch.as_tensor(w_adc) * adc[vrt]).sum() + (torch.as_tensor(w_z) * z[vrt]).sum() + \
+        (torch.as_tensor(w_wfs) * wfs[vrt]).sum()
+    L.backward()
+    g_ref = th.grad.detach().numpy()
+    g = sim.backward(st, torch.from_numpy(w_adc.astype(np.float32)), torch.from_numpy(w_z.astype(np.float32)),
+                     torch.from_numpy(w_wfs.astype(np.float32))).cpu().numpy()
+    print("gpu ", g)
+    print("ref ", g_ref)
+    mask = np.abs(g_ref) > 0
+    err = _rel(g[mask], g_ref[mask])
+    print("rel err", err)
+    assert err.max() < GRAD_TOL
+
+
+def test_autograd_through_waveforms(params4, lut, c1):
+    """A loss on HipLUTSimulation.predict's waveforms back-propagates to the parameters, equal to
+    Simulator.backward with the same waveform cotangent."""
+    from larndsim_amd.sim import Simulator
+    from larndsim_amd.strategies import HipLUTSimulation
+
+    tr, f = c1
+    strat = HipLUTSimulation(lut)
+    names = ("Ab", "kb", "eField", "lifetime", "vdrift", "long_diff", "tran_diff")
+    th = {n: torch.tensor(float(getattr(params4, n)), dtype=torch.float64, requires_grad=True) for n in names}
+    out = strat.predict(params4.replace(**th), torch.from_numpy(tr).cuda(), f)
+    w = torch.linspace(-1, 1, out["wfs"].numel(), device="cuda").reshape(out["wfs"].shape)
+    (w * out["wfs"]).sum().backward()
+    g = np.array([float(th[n].grad) for n in names])
+    sim = Simulator(params4, lut)
+    st = sim.forward(params4, tr, f, want_tape=True)
+    d_adc = torch.zeros(st.max_rows * sim.H)
+    g2 = sim.backward(st, d_adc, None, w).cpu().numpy()
+    print("autograd", g, "direct", g2, flush=True)
+    assert np.allclose(g, g2, rtol=1e-6, atol=0)
+    assert np.abs(g).max() > 0
