@@ -283,6 +283,71 @@ __device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__
   }
 }
 
+// The same window sums for a dense G (one breakpoint per tick, G(o) = c[o]: the telescoped form is
+// then no cheaper than the direct one), split over a group of GL lanes working on one pair: lane l
+// of the group takes the window's samples j = l, l + GL, ... straight from the float windows X_k
+// (the forward's own samples, whose prefix sums CX_k are); y1 = sum_{o in [lo1, hi1]} G(o) X[o - S],
+// y0 = sum_{o in [lo0, hi0]} G(o) X[o - S + 1].  Each lane returns its own share: the callers'
+// results are linear in (y1, y0), so the shares are added by the block's closing reduction, with no
+// cross-lane step here.
+template <int NT, int GL>
+__device__ __forceinline__ void g_dots_lanes(const RowG& G, const float* __restrict__ X, int64_t xs,
+                                             const PairCtx& c, int L, double* y1, double* y0) {
+  const int gl = threadIdx.x & (GL - 1);
+#pragma unroll
+  for (int k = 0; k < NT; ++k) y1[k] = y0[k] = 0.0;
+  for (int j = gl; j < L; j += GL) {
+    const int o1 = c.S + j, o0 = c.S - 1 + j;
+    const double g1 = (o1 >= c.lo1 && o1 <= c.hi1) ? (double)G.c[o1] : 0.0;
+    const double g0 = (o0 >= c.lo0 && o0 <= c.hi0) ? (double)G.c[o0] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const double x = (double)X[k * xs + j];
+      y1[k] += g1 * x;
+      y0[k] += g0 * x;
+    }
+  }
+}
+
+// The three templates m - 1, m, m + 1 of a diffusion-bin cell at once, from the interleaved
+// windows X3[j] = (T_{m-1}, T_m, T_{m+1})[j] (one 12-byte load per sample).
+template <int GL>
+__device__ __forceinline__ void g_dots_lanes3(const RowG& G, const float3* __restrict__ X3, const PairCtx& c, int L,
+                                              double* y1, double* y0) {
+  const int gl = threadIdx.x & (GL - 1);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) y1[k] = y0[k] = 0.0;
+  for (int j = gl; j < L; j += GL) {
+    const int o1 = c.S + j, o0 = c.S - 1 + j;
+    const double g1 = (o1 >= c.lo1 && o1 <= c.hi1) ? (double)G.c[o1] : 0.0;
+    const double g0 = (o0 >= c.lo0 && o0 <= c.hi0) ? (double)G.c[o0] : 0.0;
+    const float3 x = X3[j];
+    y1[0] += g1 * (double)x.x;
+    y0[0] += g0 * (double)x.x;
+    y1[1] += g1 * (double)x.y;
+    y0[1] += g0 * (double)x.y;
+    y1[2] += g1 * (double)x.z;
+    y0[2] += g0 * (double)x.z;
+  }
+}
+
+// GL = 0: one thread per pair, telescoped over the prefix sums CX; else GL lanes per pair over the
+// windows X (stride xs), each returning its share.
+template <int NT, int GL>
+__device__ __forceinline__ void g_dots_any(const RowG& G, const double* __restrict__ CX, int64_t ts,
+                                           const float* __restrict__ X, int64_t xs, const PairCtx& c, int L, double* y1,
+                                           double* y0) {
+  if constexpr (GL > 0) g_dots_lanes<NT, GL>(G, X, xs, c, L, y1, y0);
+  else g_dots<NT>(G, CX, ts, c, L, y1, y0);
+}
+
+// Weight of the terms that do not come from the window sums: 1 on one lane of each pair's group.
+template <int GL>
+__device__ __forceinline__ double lead_lane() {
+  if constexpr (GL > 0) return (threadIdx.x & (GL - 1)) == 0 ? 1.0 : 0.0;
+  else return 1.0;
+}
+
 // Distinct LUT cells of the diffusion bins that land on the row along one axis (pixel offset a):
 // bins with the same cell have their weights summed (the bin weights factorise, w = wx wy, and
 // every other factor depends on the cell only), so each distinct (cx, cy) is visited once.
@@ -341,24 +406,26 @@ __device__ __forceinline__ void pair_ctx(const Dev& d, int S, float ne, float f,
 }
 
 // The neighbour stripe of pixel offset (a, b) != (0, 0) of a segment at sub-bins (ux, uy):
-// d loss / d n_e (gN) and / d f (gF).
+// d loss / d n_e (gN) and / d f (gF).  GL > 0: this lane's share of a GL-lane group's pair.
+template <int GL = 0>
 __device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, int ux, int uy, const PairCtx& c, int a,
                                                int b, const RowG& G, double& gN, double& gF) {
   const int L = d.L, nsb = d.nsb;
   const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
   double y1, y0;
-  g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, c, L, &y1, &y0);
+  g_dots_any<1, GL>(G, tb.CTN + sidx * (L + 1), 0, tb.TN + sidx * L, 0, c, L, &y1, &y0);
   const float2 D = tb.DN[(int64_t)(d.nt - L - c.S) * d.lx * d.ly + sidx];  // tick-major, k0 = nt - L - S
-  const double corr = (double)D.x - c.f * (double)D.y;
+  const double u = lead_lane<GL>();
+  const double corr = u * ((double)D.x - c.f * (double)D.y), dy = u * (double)D.y;
   gN += (1.0 - c.f) * y1 + c.f * y0 + c.gc * corr;
-  gF += c.ne * (y0 - y1 + (c.gc1 - c.gc0) * corr - c.gc * (double)D.y);
+  gF += c.ne * (y0 - y1 + (c.gc1 - c.gc0) * corr - c.gc * dy);
 }
 
 // The diffusion bins of x-bin i (and every later x-bin with the same LUT cell, merged into it)
 // landing on pixel offset (a, b), |a|, |b| <= 1: d loss / d (n_e, f, sigma_T, lambda) summed over
 // the y bins (only y-bin jsel when jsel >= 0).  false (nothing added) when bin i misses column a
 // or an earlier x-bin has its cell.
-template <int NB>
+template <int NB, int GL = 0>
 __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, const SegRec& R, const AdjRec* __restrict__ Ap,
                                              const PairCtx& c, int a, int b, int i, int jsel, const RowG& G, double& gN,
                                              double& gF, double& gS, double& gL) {
@@ -387,14 +454,16 @@ __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, con
   const double la = (double)R.la, lb = (double)R.lb, lc = (double)R.lc;
   const double da = Ap->dl[0], db = Ap->dl[1], dc = Ap->dl[2];
   const double f = c.f, ne = c.ne;
+  const double u = lead_lane<GL>();
 #pragma unroll
   for (int jj = 0; jj < NB; ++jj) {
     if (!oky[jj] || (jsel >= 0 && jj != jsel)) continue;
     const int64_t cell = ((int64_t)R.idx * cm + cx) * cm + cyv[jj];
     double y1[3], y0[3];
-    g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c, L, y1, y0);
+    if constexpr (GL > 0) g_dots_lanes3<GL>(G, tb.TM3 + cell * kWinPad + 2, c, L, y1, y0);  // window at p = 2
+    else g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c, L, y1, y0);
     const float2 D = tb.DM[(int64_t)R.k0 * d.nm * cm * cm + cell];  // tick-major
-    const double corr = (double)D.x - f * (double)D.y;
+    const double corr = u * ((double)D.x - f * (double)D.y), dy = u * (double)D.y;
     const double wxy = (double)WXi * (double)WY[jj];
     const double dwxy = (double)DWXi * (double)WY[jj] + (double)WXi * (double)DWY[jj];
     const double Y1 = la * y1[0] + lb * y1[1] + lc * y1[2];
@@ -402,7 +471,7 @@ __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, con
     const double gn = (1.0 - f) * Y1 + f * Y0 + c.gc * corr;  // d loss / d n_bin per unit weight
     gN += gn * wxy;
     gS += gn * ne * dwxy;
-    gF += ne * wxy * (Y0 - Y1 + (c.gc1 - c.gc0) * corr - c.gc * (double)D.y);
+    gF += ne * wxy * (Y0 - Y1 + (c.gc1 - c.gc0) * corr - c.gc * dy);
     gL += ne * wxy * ((1.0 - f) * (da * y1[0] + db * y1[1] + dc * y1[2]) + f * (da * y0[0] + db * y0[1] + dc * y0[2]));
   }
   return true;
@@ -421,6 +490,13 @@ __device__ __forceinline__ void add_chain(const AdjRec* __restrict__ Ap, double 
     g[6] += gS * Ap->fS[1];
   }
 }
+
+#ifndef LSIM_DENSE_LANES
+#define LSIM_DENSE_LANES 8
+#endif
+constexpr int kDenseLanes = LSIM_DENSE_LANES;  // lanes per pair item of the dense (waveform-cotangent) adjoint
+static_assert(kDenseLanes > 0 && kDenseLanes <= 64 && (kDenseLanes & (kDenseLanes - 1)) == 0,
+              "lanes per dense item: a power of two up to the wave");
 
 #ifndef LSIM_ADJ_BIN_ITEMS
 #define LSIM_ADJ_BIN_ITEMS 0
@@ -531,7 +607,41 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
   constexpr int NX = kAdjBinItems ? NB * NB : NB;
   constexpr int NI = NEAR ? NX + 1 : 1;
   int cur = 0;
-  if (!NEAR) {
+  const int lane = t & (kWave - 1), wid = t / kWave;
+  if (DENSE) {
+    // kDenseLanes lanes per item, splitting its windows' samples; every lane adds its share
+    constexpr int GL = kDenseLanes;
+    for (int it = t / GL; it < tot * NI; it += NT / GL) {
+      const int i = it / NI, part = it - i * NI;
+      while (i >= c_cum[cur + 1]) ++cur;
+      const int s = c_beg[cur] + (i - c_cum[cur]);
+      const int ab = c_ab[cur];
+      const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+      double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
+      PairCtx pc;
+      if (!NEAR) {
+        const int4 P = *reinterpret_cast<const int4*>(prs + s);
+        const float ne = __int_as_float(P.y);
+        if (ne == 0.f) continue;
+        pair_ctx(d, P.x, ne, __int_as_float(P.z), G, pc);
+        stripe_adjoint<GL>(d, tb, (P.w >> 8) & 255, (P.w >> 16) & 255, pc, a, b, G, gN, gF);
+      } else {
+        const SegRec R = rec[s];
+        if (R.ne == 0.f) continue;
+        const int S = d.nt - d.L - R.k0;
+        if (pair_dead(S, G)) continue;
+        pair_ctx(d, S, R.ne, R.f, G, pc);
+        if (part == NX) {
+          if ((a == 0 && b == 0) || a < -d.nn || a > d.nn || b < -d.nn || b > d.nn) continue;
+          stripe_adjoint<GL>(d, tb, R.uxy & 0xffff, R.uxy >> 16, pc, a, b, G, gN, gF);
+        } else if (!xbin_adjoint<NB, GL>(d, tb, R, adj + s, pc, a, b, kAdjBinItems ? part / NB : part,
+                                           kAdjBinItems ? part % NB : -1, G, gN, gF, gS, gL)) {
+          continue;
+        }
+      }
+      add_chain(adj + s, gN, gF, gS, gL, NEAR, g);
+    }
+  } else if (!NEAR) {
     // far pairs: the stripe only.  Each thread fetches its next pair's record (the first 16 bytes
     // of the pair record) and chain factors while the current pair's table loads are in flight.
     struct FarItem {
@@ -570,7 +680,7 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
       x = y;
     }
   }
-  for (int it = t; NEAR && it < tot * NI; it += NT) {
+  for (int it = t; !DENSE && NEAR && it < tot * NI; it += NT) {
     const int i = it / NI, part = it - i * NI;
     while (i >= c_cum[cur + 1]) ++cur;
     const int s = c_beg[cur] + (i - c_cum[cur]);
@@ -592,7 +702,6 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
     }
     add_chain(adj + s, gN, gF, gS, gL, NEAR, g);
   }
-  const int lane = t & (kWave - 1), wid = t / kWave;
 #pragma unroll
   for (int k = 0; k < kNGrad; ++k) {
     const double v = wave_sum(g[k]);
