@@ -622,6 +622,35 @@ def main():
             loss="mse_adc vs target at perturbed parameters", h_rel=1e-3, grads=check,
             note="central differences of the full loss include the discrete self-trigger / tick decisions; "
                  "the adjoint (like reverse-mode AD through the reference) holds them fixed")
+
+        # the same check with a loss on the waveforms, which holds no hit or tick decision
+        def wfs_loss_at(vals, grad=False):
+            th = {n: torch.tensor(vals[n], dtype=torch.float64, requires_grad=grad) for n in names}
+            out = strat.predict(params.replace(**th), gtr, gf)
+            u, inv = torch.unique(torch.cat([out["unique_pixels"], tgt["unique_pixels"]]), return_inverse=True)
+            n_out = out["unique_pixels"].shape[0]
+            A = torch.zeros(len(u), out["wfs"].shape[1], dtype=torch.float64, device=dev)
+            B = torch.zeros_like(A)
+            A = A.index_add(0, inv[:n_out], out["wfs"].double())
+            B = B.index_add(0, inv[n_out:], tgt["wfs"].double())
+            L = ((A - B) ** 2).sum() / (B ** 2).sum()
+            if grad:
+                L.backward()
+                return {n: float(th[n].grad) if th[n].grad is not None else 0.0 for n in names}
+            return float(L)
+
+        g_adj = wfs_loss_at(v0, grad=True)
+        check = {}
+        for n in names:
+            h = 1e-3 * abs(v0[n])
+            with torch.no_grad():
+                fd = (wfs_loss_at({**v0, n: v0[n] + h}) - wfs_loss_at({**v0, n: v0[n] - h})) / (2 * h)
+            check[n] = dict(adjoint=g_adj[n], central_fd=fd,
+                            rel_err=abs(g_adj[n] - fd) / abs(fd) if fd != 0 else None)
+        extra["gradient_check_50k_waveform"] = dict(
+            loss="sum over pixels and ticks of (wfs - wfs_target)^2 / sum wfs_target^2, target at perturbed parameters",
+            h_rel=1e-3, grads=check)
+        log(f"[bench] waveform-loss gradient check: {check}")
         del gtr, tgt
 
     cpu = None

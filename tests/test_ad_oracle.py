@@ -43,3 +43,50 @@ def test_gradient_matches_central_differences(frozen):
             assert g[i] == 0.0 and fd == 0.0
             continue
         assert g[i] == pytest.approx(fd, rel=2e-5, abs=1e-9 * abs(fd) + 1e-12), A.GRAD_PARAMS[i]
+
+
+def _wfs_loss(U, W, tU, tW):
+    u, inv = np.unique(np.concatenate([U, tU]), return_inverse=True)
+    a = torch.zeros((len(u), W.shape[1]), dtype=torch.float64).index_add(0, torch.as_tensor(inv[: len(U)]), W)
+    b = torch.zeros_like(a).index_add(0, torch.as_tensor(inv[len(U):]), torch.as_tensor(tW))
+    return ((a - b) ** 2).sum() / (b ** 2).sum()
+
+
+def test_waveform_loss_central_differences(params4, lut, c1, tables):
+    """A loss on the waveforms holds no hit decision.  Central differences of the whole chain (the
+    NumPy oracle, decisions free) agree with the frozen chain's gradient at h = 1e-4 for every
+    parameter that leaves the drift time alone.  eField moves every segment's drift time, and the
+    waveform is linear in it between whole ticks (a kink at each tick crossing): differences over
+    an interval holding crossings average the slopes on both sides, and shrinking h brings them to
+    the gradient."""
+    from oracle import oracle as O
+
+    tr, f = c1
+    tr = tr[:120]
+    p = params4
+    names = A.GRAD_PARAMS
+    v0 = {n: float(getattr(p, n)) for n in names}
+    scale = {"Ab": 1.02, "kb": 0.97, "eField": 1.01, "lifetime": 0.9, "vdrift": 1.0, "long_diff": 1.1,
+             "tran_diff": 0.92}
+    t = O.simulate(p.replace(**{n: v0[n] * scale[n] for n in names}), lut, tr, f, tables=tables)
+    tU, tW = t["unique_pixels"], np.asarray(t["wfs"], np.float64)
+    fr = A.Frozen(p, lut, tr, f, tables=tables)
+    th = fr.theta0().requires_grad_(True)
+    _wfs_loss(fr.U, fr.forward(th)[2], tU, tW).backward()
+    g = th.grad.numpy()
+
+    def fd(n, hr):
+        h = hr * abs(v0[n])
+        out = []
+        for s in (1, -1):
+            o = O.simulate(p.replace(**{n: v0[n] + s * h}), lut, tr, f, tables=tables)
+            out.append(float(_wfs_loss(o["unique_pixels"], torch.as_tensor(np.asarray(o["wfs"], np.float64)), tU, tW)))
+        return (out[0] - out[1]) / (2 * h)
+
+    for n in ("Ab", "kb", "lifetime", "long_diff", "tran_diff"):
+        i = names.index(n)
+        assert g[i] == pytest.approx(fd(n, 1e-4), rel=1e-4), n
+    i = names.index("eField")
+    errs = [abs(fd("eField", hr) - g[i]) / abs(g[i]) for hr in (1e-3, 1e-6)]
+    print("eField relative difference at h = 1e-3, 1e-6:", errs)
+    assert errs[1] < 1e-3 and errs[1] < errs[0] / 5

@@ -198,3 +198,50 @@ def test_autograd_through_waveforms(params4, lut, c1):
     print("autograd", g, "direct", g2, flush=True)
     assert np.allclose(g, g2, rtol=1e-6, atol=0)
     assert np.abs(g).max() > 0
+
+
+def _waveform_mse(out, tgt):
+    """sum over pixels and ticks of (w - w_target)^2 / sum w_target^2, rows matched by pixel id (a
+    pixel missing on one side counts as zeros)."""
+    ids, w = out["unique_pixels"], out["wfs"]
+    tids, tw = tgt["unique_pixels"], tgt["wfs"]
+    u, inv = torch.unique(torch.cat([ids, tids]), return_inverse=True)
+    A = torch.zeros(len(u), w.shape[1], dtype=torch.float64, device=w.device).index_add(0, inv[: len(ids)], w.double())
+    B = torch.zeros(len(u), w.shape[1], dtype=torch.float64, device=w.device).index_add(0, inv[len(ids):], tw.double())
+    return ((A - B) ** 2).sum() / (B ** 2).sum()
+
+
+def test_waveform_loss_matches_central_differences(params4, lut, c1):
+    """A loss on the waveforms has no hit decision in it, so the adjoint must equal central
+    differences (h = 1e-3 relative) of the whole loss, through the float32 forward, for every
+    parameter that leaves the drift time alone.  eField moves the drift times, and the waveforms
+    are linear in them between whole ticks: over +-h many segments cross a tick
+    and the differences average the slopes on both sides, so eField is held to 5e-2 here (the
+    oracle test of the same loss shows the differences closing on the gradient as h shrinks)."""
+    from larndsim_amd.strategies import HipLUTSimulation
+
+    tr, f = c1
+    strat = HipLUTSimulation(lut)
+    trd = torch.from_numpy(tr).cuda()
+    names = ("Ab", "kb", "eField", "lifetime", "vdrift", "long_diff", "tran_diff")
+    v0 = {n: float(getattr(params4, n)) for n in names}
+    scale = {"Ab": 1.02, "kb": 0.97, "eField": 1.01, "lifetime": 0.9, "vdrift": 1.0, "long_diff": 1.1, "tran_diff": 0.92}
+    with torch.no_grad():
+        tgt = strat.predict(params4.replace(**{n: v0[n] * scale[n] for n in names}), trd, f)
+    th = {n: torch.tensor(v0[n], dtype=torch.float64, requires_grad=True) for n in names}
+    _waveform_mse(strat.predict(params4.replace(**th), trd, f), tgt).backward()
+    g = np.array([float(th[n].grad) if th[n].grad is not None else 0.0 for n in names])
+    fd = np.zeros(len(names))
+    with torch.no_grad():
+        for i, n in enumerate(names):
+            h = 1e-3 * abs(v0[n])
+            lp = float(_waveform_mse(strat.predict(params4.replace(**{n: v0[n] + h}), trd, f), tgt))
+            lm = float(_waveform_mse(strat.predict(params4.replace(**{n: v0[n] - h}), trd, f), tgt))
+            fd[i] = (lp - lm) / (2 * h)
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
+    print("adjoint", g, "\ncentral", fd, "\nrel", rel, flush=True)
+    assert g[4] == 0.0 and fd[4] == 0.0
+    smooth = [names.index(n) for n in ("Ab", "kb", "lifetime", "long_diff", "tran_diff")]
+    assert all(abs(fd[i]) * abs(v0[names[i]]) > 1e-6 for i in smooth)  # d loss / d log(param) above noise
+    assert rel[smooth].max() < GRAD_TOL
+    assert rel[names.index("eField")] < 5e-2
