@@ -9,7 +9,8 @@ Also measured in the same run (extra keys): forward+backward (7-parameter gradie
 event, the plain-forward finite-difference gradient baseline (15 forwards), the batched forward
 of 128 uneven events in one hipGraph replay, the batched forward+backward of 32 uneven events in
 one hipGraph replay against the per-event forward-then-finite-difference gradient, a 200-step
-Adam fit of the 7 parameters on those 32 events (steps/s, parameter recovery), and the CPU
+Adam fit of the 7 parameters on those 32 events (steps/s, parameter recovery) with the hit loss
+and a 100-step one with a loss on the waveforms, and the CPU
 baseline (the float32 NumPy restatement of the reference algorithm on a bounded sample, on the
 host cores).
 
@@ -350,6 +351,8 @@ def main():
     ap.add_argument("--no-batched-bwd", action="store_true", help="skip the batched forward+backward of 32 events")
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--fit-steps", type=int, default=200, help="Adam steps of the 32-event gradient fit (0: skip)")
+    ap.add_argument("--fit-wfs-steps", type=int, default=100,
+                    help="Adam steps of the same fit with a loss on the waveforms (0: skip)")
     ap.add_argument("--copy-calibrate", action="store_true")
     ap.add_argument("--no-chop", action="store_true", help="skip the track-chopping leg")
     ap.add_argument("--chop-only", action="store_true", help="profiling: only the track-chopping leg")
@@ -585,6 +588,45 @@ def main():
                             loss="mse_adc: hit-cloud MMD (sigma 1 cm) + relative total charge")
         log(f"[bench] fit {1.0 / dtfit:.2f} steps/s; loss {fitter.history['loss'][0]:.4g} -> "
             f"{fitter.history['loss'][-1]:.4g}; rel err start {rel0} end {rel1}")
+
+        # the same fit on a loss over the waveforms (no hit decision in it), through the dense adjoint
+        if a.fit_wfs_steps > 0:
+            from larndsim_amd.strategies import LossStrategy
+
+            class WaveformMSE(LossStrategy):
+                """sum over pixels and ticks of (wfs - wfs_target)^2 / sum wfs_target^2, rows matched by
+                pixel id (which carries the event)."""
+
+                def compute(self, params, prediction, target):
+                    ids, tids = prediction["unique_pixels"], target["unique_pixels"]
+                    u, inv = torch.unique(torch.cat([ids, tids]), return_inverse=True)
+                    A = torch.zeros(len(u), prediction["wfs"].shape[1], dtype=torch.float64, device=dev)
+                    B = torch.zeros_like(A).index_add(0, inv[ids.shape[0]:], target["wfs"].double())
+                    A = A.index_add(0, inv[: ids.shape[0]], prediction["wfs"].double())
+                    return ((A - B) ** 2).sum() / (B ** 2).sum(), {}
+
+            log("[bench] gradient fit on 32 events, waveform loss")
+            fitter = GradientDescentFitter(strat, WaveformMSE(), params, names, lr=2e-2, max_clip_norm_val=1.0,
+                                           lr_scheduler="exponential_decay", lr_kw=dict(decay_rate=0.97))
+            fitter.step(ftr, ff, target)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(a.fit_wfs_steps - 1):
+                fitter.step(ftr, ff, target)
+                if i % 25 == 24:
+                    log(f"[bench] waveform fit step {i + 2}: loss {fitter.history['loss'][-1]:.4g}")
+            torch.cuda.synchronize()
+            dtw = (time.perf_counter() - t0) / max(a.fit_wfs_steps - 1, 1)
+            final = fitter.physical()
+            rel1 = {n: abs(final[n] / float(getattr(target_params, n)) - 1) for n in names}
+            extra["fit_waveform"] = dict(events=32, segments=int(ft.shape[0]), steps=a.fit_wfs_steps,
+                                         steps_per_s=1.0 / dtw, segments_per_s=ft.shape[0] / dtw,
+                                         loss_first=fitter.history["loss"][0], loss_last=fitter.history["loss"][-1],
+                                         rel_error_start=rel0, rel_error_end=rel1,
+                                         optimizer="as the fit above",
+                                         loss="sum (wfs - wfs_target)^2 / sum wfs_target^2, rows matched by pixel id")
+            log(f"[bench] waveform fit {1.0 / dtw:.2f} steps/s; loss {fitter.history['loss'][0]:.4g} -> "
+                f"{fitter.history['loss'][-1]:.4g}; rel err end {rel1}")
         del ftr, target, fitter
 
     # ---- gradient check on the 50k-segment event: adjoint vs central differences of the full loss ----
