@@ -251,22 +251,26 @@ __device__ inline void seg_physics(const Dev& d, const SegIn& in, int64_t i, Phy
 
 // Transverse weights along one axis (detsim_jax.py:332-341) and their derivative in sigma.
 __device__ inline void tran_weights(const Dev& d, double x0, double st, double* W, double* dW) {
+  // every index below is a compile-time constant (the bin count nb <= kMaxNb only selects), so the
+  // edge arrays stay in registers
   double E[kMaxNb + 1], dEs[kMaxNb + 1];
-  E[0] = -1.0;
-  E[d.nb] = 1.0;
-  dEs[0] = dEs[d.nb] = 0.0;
-  const double inv = 1.0 / (M_SQRT2 * st);
-  for (int k = 1; k < d.nb; ++k) {
-    const double e = (double)(k - d.nbsym) * d.w - x0;
-    const double xi = e / (M_SQRT2 * st);
-    E[k] = erf(xi);
-    // d erf(e / (sqrt2 s)) / d s = 2/sqrt(pi) exp(-xi^2) * (-xi / s)
-    dEs[k] = M_2_SQRTPI * exp(-xi * xi) * (-xi / st);
-    (void)inv;
+#pragma unroll
+  for (int k = 0; k <= kMaxNb; ++k) {
+    if (k == 0 || k >= d.nb) {
+      E[k] = k == 0 ? -1.0 : 1.0;
+      dEs[k] = 0.0;
+    } else {
+      const double e = (double)(k - d.nbsym) * d.w - x0;
+      const double xi = e / (M_SQRT2 * st);
+      E[k] = erf(xi);
+      // d erf(e / (sqrt2 s)) / d s = 2/sqrt(pi) exp(-xi^2) * (-xi / s)
+      dEs[k] = M_2_SQRTPI * exp(-xi * xi) * (-xi / st);
+    }
   }
-  for (int k = 0; k < d.nb; ++k) {
-    W[k] = 0.5 * (E[k + 1] - E[k]);
-    if (dW) dW[k] = 0.5 * (dEs[k + 1] - dEs[k]);
+#pragma unroll
+  for (int k = 0; k < kMaxNb; ++k) {
+    W[k] = k < d.nb ? 0.5 * (E[k + 1] - E[k]) : 0.0;
+    if (dW) dW[k] = k < d.nb ? 0.5 * (dEs[k + 1] - dEs[k]) : 0.0;
   }
 }
 

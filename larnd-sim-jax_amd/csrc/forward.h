@@ -96,44 +96,86 @@ __global__ void k_cell_keys(Dev d, SegIn in, const uint32_t* __restrict__ vals, 
 // C[k0+1] - C[k0]), ab = (a + n)(2n + 1) + (b + n).  The D tables are tick-major, so all of one
 // segment's entries lie in the tick-k0 plane; the waveform kernels then read dtot[ab][cell range]
 // contiguously.
+//
+// The table loads are issued in batches (kCorrBatch stripe entries, then one x-bin row of
+// diffusion bins at a time) so that each thread has several in flight; each entry still receives
+// its terms in the same order (its stripe term, then its diffusion bins by x-bin, y-bin).
+constexpr int kCorrBatch = 9;
+
 template <int NB>
 __device__ __forceinline__ void seg_corrections(const Dev& d, const float2* __restrict__ DN, const float2* __restrict__ DM,
                                                 const SegRec& R, int ux, int uy, int64_t i, int64_t n,
                                                 float* __restrict__ dtot) {
   constexpr int SYM = (NB - 1) / 2;
-  const int W = 2 * d.ne + 1, nsb = d.nsb, cm = d.cm;
+  const int W = 2 * d.ne + 1, WW = W * W, nsb = d.nsb, cm = d.cm;
   const float f = R.f;
   const float2* DNk = DN + (int64_t)R.k0 * d.lx * d.ly;
   const float2* DMk = DM + ((int64_t)R.k0 * d.nm + R.idx) * cm * cm;
-  for (int ab = 0; ab < W * W; ++ab) {
-    const int a = ab / W - d.ne, b = ab % W - d.ne;
-    float delta = 0.f;
-    if ((a != 0 || b != 0) && a >= -d.nn && a <= d.nn && b >= -d.nn && b <= d.nn) {
-      const float2 D = DNk[offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb)];
-      delta = R.ne * (D.x - f * D.y);
+  float dn[9];  // entries (a, b) in [-1, 1]^2, index (a + 1) * 3 + (b + 1)
+  // stripe terms of every entry
+  for (int ab0 = 0; ab0 < WW; ab0 += kCorrBatch) {
+    float2 D[kCorrBatch];
+#pragma unroll
+    for (int u = 0; u < kCorrBatch; ++u) {
+      const int ab = ab0 + u;
+      const int a = ab / W - d.ne, b = ab % W - d.ne;
+      const bool on = ab < WW && (a != 0 || b != 0) && a >= -d.nn && a <= d.nn && b >= -d.nn && b <= d.nn;
+      D[u] = on ? DNk[offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb)] : make_float2(0.f, 0.f);
     }
-    if (a >= -1 && a <= 1 && b >= -1 && b <= 1) {
+    float dl[kCorrBatch];  // all of the batch's values before its stores: no store between two uses
 #pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        const int vx = ux + k - SYM;
-        const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
-        if (dx != a) continue;
-        const float wxs = R.ne * R.wx[k];
-        const int cx = offset_bin(vx - dx * nsb, 0, nsb);
+    for (int u = 0; u < kCorrBatch; ++u) {
+      const int ab = ab0 + u;
+      const int a = ab / W - d.ne, b = ab % W - d.ne;
+      const bool on = ab < WW && (a != 0 || b != 0) && a >= -d.nn && a <= d.nn && b >= -d.nn && b <= d.nn;
+      dl[u] = on ? R.ne * (D[u].x - f * D[u].y) : 0.f;
+    }
 #pragma unroll
-        for (int jj = 0; jj < NB; ++jj) {
-          const int vy = uy + jj - SYM;
-          const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
-          if (dy != b) continue;
-          const float c = wxs * R.wy[jj];
-          if (c != 0.f) {
-            const float2 D = DMk[cx * cm + offset_bin(vy - dy * nsb, 0, nsb)];
-            delta += c * (D.x - f * D.y);
-          }
-        }
+    for (int u = 0; u < kCorrBatch; ++u) {
+      const int ab = ab0 + u;
+      if (ab >= WW) break;
+      const int a = ab / W - d.ne, b = ab % W - d.ne;
+      const float delta = dl[u];
+      if (a >= -1 && a <= 1 && b >= -1 && b <= 1) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+          if (e == (a + 1) * 3 + (b + 1)) dn[e] = delta;
+      } else {
+        dtot[(int64_t)ab * n + i] = delta;
       }
     }
-    dtot[(int64_t)ab * n + i] = delta;
+  }
+  // diffusion bins of the idx template, one x-bin row of loads at a time
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int vx = ux + k - SYM;
+    const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
+    const float wxs = R.ne * R.wx[k];
+    const int cx = offset_bin(vx - dx * nsb, 0, nsb);
+    float2 D[NB];
+    float c[NB];
+    int e[NB];
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+      const int vy = uy + jj - SYM;
+      const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
+      c[jj] = wxs * R.wy[jj];
+      e[jj] = (dx + 1) * 3 + (dy + 1);
+      D[jj] = c[jj] != 0.f ? DMk[cx * cm + offset_bin(vy - dy * nsb, 0, nsb)] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+      if (c[jj] == 0.f) continue;
+      const float v = c[jj] * (D[jj].x - f * D[jj].y);
+#pragma unroll
+      for (int q = 0; q < 9; ++q)
+        if (q == e[jj]) dn[q] += v;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const int a = q / 3 - 1, b = q % 3 - 1;
+    dtot[(int64_t)((a + d.ne) * W + (b + d.ne)) * n + i] = dn[q];
   }
 }
 
