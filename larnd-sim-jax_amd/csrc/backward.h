@@ -290,21 +290,54 @@ __device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__
 // y0 = sum_{o in [lo0, hi0]} G(o) X[o - S + 1].  Each lane returns its own share: the callers'
 // results are linear in (y1, y0), so the shares are added by the block's closing reduction, with no
 // cross-lane step here.
+//
+// kLaneUnroll samples per lane are loaded before any is used, with no branch around the loads, so
+// they are in flight together: a sample past the window (j >= L) reads sample L - 1 and a tick
+// outside the row reads a clamped entry of G, and both get weight 0 (j >= L puts o past hi1 and
+// hi0).  Each lane still adds its samples in increasing j.
+constexpr int kLaneUnroll = 4;
+
+struct LaneTick {
+  int jl;         // sample index to load (clamped into the window)
+  float c1, c0;   // G at o1 = S + j and o0 = S - 1 + j (clamped ticks)
+  bool in1, in0;  // o1 in [lo1, hi1], o0 in [lo0, hi0] (false past the window)
+};
+
+__device__ __forceinline__ LaneTick lane_tick(const RowG& G, const PairCtx& c, int L, int j) {
+  LaneTick t;
+  const int o1 = c.S + j, o0 = o1 - 1, top = G.np - 1;
+  t.jl = j < L ? j : L - 1;
+  t.in1 = j < L && o1 >= c.lo1 && o1 <= c.hi1;
+  t.in0 = j < L && o0 >= c.lo0 && o0 <= c.hi0;
+  t.c1 = G.c[o1 < 0 ? 0 : (o1 > top ? top : o1)];
+  t.c0 = G.c[o0 < 0 ? 0 : (o0 > top ? top : o0)];
+  return t;
+}
+
 template <int NT, int GL>
 __device__ __forceinline__ void g_dots_lanes(const RowG& G, const float* __restrict__ X, int64_t xs,
                                              const PairCtx& c, int L, double* y1, double* y0) {
   const int gl = threadIdx.x & (GL - 1);
 #pragma unroll
   for (int k = 0; k < NT; ++k) y1[k] = y0[k] = 0.0;
-  for (int j = gl; j < L; j += GL) {
-    const int o1 = c.S + j, o0 = c.S - 1 + j;
-    const double g1 = (o1 >= c.lo1 && o1 <= c.hi1) ? (double)G.c[o1] : 0.0;
-    const double g0 = (o0 >= c.lo0 && o0 <= c.hi0) ? (double)G.c[o0] : 0.0;
+  for (int j0 = gl; j0 < L; j0 += kLaneUnroll * GL) {
+    LaneTick tk[kLaneUnroll];
+    float x[kLaneUnroll][NT];
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
-      const double x = (double)X[k * xs + j];
-      y1[k] += g1 * x;
-      y0[k] += g0 * x;
+    for (int u = 0; u < kLaneUnroll; ++u) {
+      tk[u] = lane_tick(G, c, L, j0 + u * GL);
+#pragma unroll
+      for (int k = 0; k < NT; ++k) x[u][k] = X[k * xs + tk[u].jl];
+    }
+#pragma unroll
+    for (int u = 0; u < kLaneUnroll; ++u) {
+      const double g1 = tk[u].in1 ? (double)tk[u].c1 : 0.0;
+      const double g0 = tk[u].in0 ? (double)tk[u].c0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        y1[k] += g1 * (double)x[u][k];
+        y0[k] += g0 * (double)x[u][k];
+      }
     }
   }
 }
@@ -317,17 +350,25 @@ __device__ __forceinline__ void g_dots_lanes3(const RowG& G, const float3* __res
   const int gl = threadIdx.x & (GL - 1);
 #pragma unroll
   for (int k = 0; k < 3; ++k) y1[k] = y0[k] = 0.0;
-  for (int j = gl; j < L; j += GL) {
-    const int o1 = c.S + j, o0 = c.S - 1 + j;
-    const double g1 = (o1 >= c.lo1 && o1 <= c.hi1) ? (double)G.c[o1] : 0.0;
-    const double g0 = (o0 >= c.lo0 && o0 <= c.hi0) ? (double)G.c[o0] : 0.0;
-    const float3 x = X3[j];
-    y1[0] += g1 * (double)x.x;
-    y0[0] += g0 * (double)x.x;
-    y1[1] += g1 * (double)x.y;
-    y0[1] += g0 * (double)x.y;
-    y1[2] += g1 * (double)x.z;
-    y0[2] += g0 * (double)x.z;
+  for (int j0 = gl; j0 < L; j0 += kLaneUnroll * GL) {
+    LaneTick tk[kLaneUnroll];
+    float3 x[kLaneUnroll];
+#pragma unroll
+    for (int u = 0; u < kLaneUnroll; ++u) {
+      tk[u] = lane_tick(G, c, L, j0 + u * GL);
+      x[u] = X3[tk[u].jl];
+    }
+#pragma unroll
+    for (int u = 0; u < kLaneUnroll; ++u) {
+      const double g1 = tk[u].in1 ? (double)tk[u].c1 : 0.0;
+      const double g0 = tk[u].in0 ? (double)tk[u].c0 : 0.0;
+      y1[0] += g1 * (double)x[u].x;
+      y0[0] += g0 * (double)x[u].x;
+      y1[1] += g1 * (double)x[u].y;
+      y0[1] += g0 * (double)x[u].y;
+      y1[2] += g1 * (double)x[u].z;
+      y0[2] += g0 * (double)x[u].z;
+    }
   }
 }
 
