@@ -295,7 +295,10 @@ __device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__
 // they are in flight together: a sample past the window (j >= L) reads sample L - 1 and a tick
 // outside the row reads a clamped entry of G, and both get weight 0 (j >= L puts o past hi1 and
 // hi0).  Each lane still adds its samples in increasing j.
-constexpr int kLaneUnroll = 4;
+#ifndef LSIM_LANE_UNROLL
+#define LSIM_LANE_UNROLL 4
+#endif
+constexpr int kLaneUnroll = LSIM_LANE_UNROLL;
 
 struct LaneTick {
   int jl;         // sample index to load (clamped into the window)
