@@ -143,6 +143,18 @@ int lsim_forward_rows(lsim_ctx *ctx, const lsim_params *p, int64_t n_seg, int32_
 int lsim_forward(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg, int32_t n_events, void *ws,
                  size_t ws_bytes, int64_t max_rows, const lsim_fwd_out *out, void *stream);
 
+/* Negative ids of the reference's pixel set (src/larndsim/sim_jax.py:716-721: sorted unique main-pixel
+ * ids plus an appended -1).  A padding segment (eventID -1) on the pixel plane has the negative id
+ * pixel2id packs for event -1 (src/larndsim/detsim_jax.py:230-244); a segment whose main pixel is
+ * off the plane has id -1.  The reference keeps a row for each distinct one; they carry no charge
+ * and no hits.  lsim_negative_rows_query gives the workspace bytes and the largest possible count
+ * (n_tpc * n_pixels_x * n_pixels_y); lsim_negative_rows writes the distinct negative main-pixel ids
+ * (not the appended -1) in ascending order to ids (at most max_ids of them) and their count to
+ * *n_ids (device int64).  Segments with eventID < -1 are skipped. */
+int lsim_negative_rows_query(lsim_ctx *ctx, size_t *ws_bytes, int64_t *max_ids);
+int lsim_negative_rows(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *seg, void *ws, size_t ws_bytes,
+                       int64_t *ids, int64_t max_ids, int64_t *n_ids, void *stream);
+
 /* Workspace bytes of lsim_fee_forward for n_rows rows. */
 int lsim_fee_query(lsim_ctx *ctx, int64_t n_rows, size_t *ws_bytes);
 

@@ -1056,6 +1056,67 @@ __global__ void k_hits(Dev d, const int64_t* __restrict__ pids, const float* __r
 }
 
 // ------------------------------------------------------------------------------------------
+// 4b. negative pixel ids of the reference's pixel set
+// ------------------------------------------------------------------------------------------
+
+// The reference's pixel set is the sorted unique main-pixel ids plus an appended -1
+// (sim_jax.py:716-721).  Besides the ids >= 0 (the rows built here) it holds negative ids:
+// a padding segment (eventID -1, optimize/dataio.py:347-359) on the pixel plane gets the id
+// pixel2id packs for event -1 (detsim_jax.py:230-244), and a segment whose main pixel is off the
+// plane gets -1.  Those rows carry no charge and no hits, so only their ids are produced here.
+// Every such id lies in [-ntpc ny nx, -1]; bit (id + ntpc ny nx) of `bits` marks it (integer
+// atomics: the bit set does not depend on the order).  eventID < -1 is invalid input and skipped.
+__global__ void k_neg_mark(Dev d, SegIn in, uint32_t* __restrict__ bits) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= in.n) return;
+  const int ev = to_int_clamped((double)ldcol(in.ev, i, in.stride));
+  if (ev < -1) return;
+  Phys ph;
+  seg_physics(d, in, i, ph);
+  const bool on_plane = ph.mpx >= 0 && ph.mpx < d.nx && ph.mpy >= 0 && ph.mpy < d.ny;
+  if (ev >= 0 && on_plane) return;
+  const int64_t span = (int64_t)d.ntpc * d.ny * d.nx;
+  const int64_t b = on_plane ? pixel_id(d, -1, ph.plane, ph.mpx, ph.mpy) + span : span - 1;
+  if (b < 0 || b >= span) return;
+  atomicOr(&bits[b >> 5], 1u << (b & 31));
+}
+
+// The marked ids in ascending order (one workgroup of kPadThreads threads: each takes a
+// contiguous run of bitmap words, counts its bits, and writes its ids after a block scan).
+constexpr int kPadThreads = 1024;
+
+__global__ __launch_bounds__(kPadThreads) void k_pad_list(const uint32_t* __restrict__ bits, int64_t span,
+                                                         int64_t* __restrict__ ids, int64_t max_ids,
+                                                         int64_t* __restrict__ n_out) {
+  __shared__ int64_t part[kPadThreads];
+  const int t = threadIdx.x;
+  const int64_t words = (span + 31) >> 5;
+  const int64_t per = (words + kPadThreads - 1) / kPadThreads;
+  const int64_t w0 = (int64_t)t * per, w1 = w0 + per < words ? w0 + per : words;
+  int64_t cnt = 0;
+  for (int64_t w = w0; w < w1; ++w) cnt += __popc(bits[w]);
+  part[t] = cnt;
+  __syncthreads();
+  for (int off = 1; off < kPadThreads; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int64_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t pos = part[t] - cnt;
+  for (int64_t w = w0; w < w1; ++w) {
+    uint32_t m = bits[w];
+    while (m) {
+      const int k = __ffs(m) - 1;
+      m &= m - 1;
+      if (pos < max_ids) ids[pos] = w * 32 + k - span;
+      ++pos;
+    }
+  }
+  if (t == kPadThreads - 1) *n_out = part[t];
+}
+
+// ------------------------------------------------------------------------------------------
 // 5. response tables (consts_jax.py:427-447), float64 arithmetic
 // ------------------------------------------------------------------------------------------
 

@@ -555,6 +555,38 @@ int lsim_forward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, in
   return lsim_forward_rows(c, p, seg->n, n_events, ws, ws_bytes, max_rows, out, stream);
 }
 
+int lsim_negative_rows_query(lsim_ctx* c, size_t* ws_bytes, int64_t* max_ids) {
+  if (!c || !ws_bytes || !max_ids) return fail(LSIM_E_ARG, "NULL argument");
+  const int64_t span = (int64_t)c->geo.n_tpc * c->geo.n_pixels_x * c->geo.n_pixels_y;
+  *ws_bytes = (size_t)((span + 31) >> 5) * sizeof(uint32_t);
+  *max_ids = span;
+  return LSIM_OK;
+}
+
+int lsim_negative_rows(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, void* ws, size_t ws_bytes,
+                       int64_t* ids, int64_t max_ids, int64_t* n_ids, void* stream) {
+  if (!c || !p || !ws || !n_ids || (max_ids > 0 && !ids)) return fail(LSIM_E_ARG, "NULL argument");
+  if (max_ids < 0) return fail(LSIM_E_ARG, "max_ids must be >= 0");
+  hipStream_t st = (hipStream_t)stream;
+  Dev d;
+  int rc = make_dev(c, p, 1, d);
+  if (rc) return rc;
+  rc = check_segments(seg, d.recomb);
+  if (rc) return rc;
+  const int64_t span = (int64_t)d.ntpc * d.nx * d.ny;
+  const int64_t words = (span + 31) >> 5;
+  if ((size_t)words * sizeof(uint32_t) > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
+  uint32_t* bits = (uint32_t*)ws;
+  fill<uint32_t>(bits, words, 0u, st);
+  if (seg->n > 0) {
+    SegIn in = make_segin(seg);
+    hipLaunchKernelGGL(k_neg_mark, dim3((unsigned)((seg->n + 255) / 256)), dim3(256), 0, st, d, in, bits);
+  }
+  hipLaunchKernelGGL(k_pad_list, dim3(1), dim3(kPadThreads), 0, st, bits, span, ids, max_ids, n_ids);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
 int lsim_fee_query(lsim_ctx* c, int64_t n_rows, size_t* bytes) {
   if (!c || !bytes) return fail(LSIM_E_ARG, "NULL argument");
   Carve cv(nullptr);
@@ -566,12 +598,16 @@ int lsim_fee_query(lsim_ctx* c, int64_t n_rows, size_t* bytes) {
   return LSIM_OK;
 }
 
-int lsim_fee_forward(lsim_ctx* c, const lsim_params* p, const float* wfs, const int64_t* pixel_ids, int64_t n_rows,
-                     void* ws, size_t ws_bytes, const lsim_fwd_out* out, void* stream) {
+}  // extern "C"
+
+// The self-trigger of caller waveforms over a workspace of lsim_fee_query(n_rows) bytes at `ws`
+// (a caller buffer, or the file-scope scratch of lsim_fee_forward_noalloc).  Every workspace
+// value is written before it is read, so the workspace needs no clearing.
+static int fee_forward_impl(lsim_ctx* c, const lsim_params* p, const float* wfs, const int64_t* pixel_ids,
+                            int64_t n_rows, void* ws, size_t ws_bytes, const lsim_fwd_out* out, hipStream_t st) {
   if (!c || !p || !out || !out->adc || !out->ticks || !ws) return fail(LSIM_E_ARG, "NULL argument");
   if (n_rows < 0) return fail(LSIM_E_ARG, "n_rows < 0");
   if (n_rows > 0 && (!wfs || !pixel_ids)) return fail(LSIM_E_ARG, "wfs / pixel_ids is NULL");
-  hipStream_t st = (hipStream_t)stream;
   Dev d;
   int rc = make_dev(c, p, 1, d);
   if (rc) return rc;
@@ -606,6 +642,13 @@ int lsim_fee_forward(lsim_ctx* c, const lsim_params* p, const float* wfs, const 
   if (out->n_hits) hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, tot, out->n_hits, (int64_t)-1, (int*)nullptr);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
+}
+
+extern "C" {
+
+int lsim_fee_forward(lsim_ctx* c, const lsim_params* p, const float* wfs, const int64_t* pixel_ids, int64_t n_rows,
+                     void* ws, size_t ws_bytes, const lsim_fwd_out* out, void* stream) {
+  return fee_forward_impl(c, p, wfs, pixel_ids, n_rows, ws, ws_bytes, out, (hipStream_t)stream);
 }
 
 int lsim_segment_records(lsim_ctx* c, int64_t n_seg, int32_t n_events, void* ws, size_t ws_bytes, void* rec_out,

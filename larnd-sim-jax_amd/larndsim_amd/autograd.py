@@ -31,6 +31,7 @@ class LUTSimFunction(torch.autograd.Function):
         others = {k: v for k, v in h.items() if k not in ("adcs", "pixel_z")}
         others["unique_pixels"] = st.out["pixel_ids"][: st.max_rows]
         others["wfs"] = st.out["wfs"][: st.max_rows]
+        others["negative_rows"] = sim.negative_rows(p, st.seg)
         ctx.others = others
         wfs = st.out["wfs"][: st.max_rows]  # a view object of its own (not the one kept above)
         return h["adcs"], h["pixel_z"], wfs
@@ -65,4 +66,8 @@ def theta_of(params):
 
 
 def needs_grad(params) -> bool:
+    """True when autograd would record the simulation: grad mode is on and one of the 7 parameters
+    is a tensor that requires a gradient (under ``torch.no_grad()`` nothing is recorded)."""
+    if not torch.is_grad_enabled():
+        return False
     return any(isinstance(getattr(params, n), torch.Tensor) and getattr(params, n).requires_grad for n in GRAD_PARAMS)

@@ -296,6 +296,20 @@ class Simulator:
                                           self._stream()))
         return grad
 
+    def negative_rows(self, params, seg):
+        """Negative ids of the reference's pixel set, on the device: ``(ids, count)`` with the
+        distinct negative main-pixel ids ascending in ``ids[:count]`` (padding segments' packed
+        event -1 ids, and -1 when a main pixel is off the plane).  Stream ordered, no host read."""
+        sz, mx = C.c_size_t(), C.c_int64()
+        _lib.check(self.lib.lsim_negative_rows_query(self._ctx, C.byref(sz), C.byref(mx)))
+        ws = torch.empty(int(sz.value), dtype=torch.uint8, device=self.device)
+        ids = torch.empty(max(int(mx.value), 1), dtype=torch.int64, device=self.device)
+        n = torch.zeros(1, dtype=torch.int64, device=self.device)
+        _lib.check(self.lib.lsim_negative_rows(self._ctx, C.byref(params_struct(params)), C.byref(seg),
+                                               C.c_void_p(ws.data_ptr()), ws.numel(), C.c_void_p(ids.data_ptr()),
+                                               ids.numel(), C.c_void_p(n.data_ptr()), self._stream()))
+        return ids, n
+
     def fee(self, params, wfs, pixel_ids):
         """Self-trigger + digitisation + compaction of caller waveforms (simulate_stochastic)."""
         _check_noise(params)
@@ -350,22 +364,36 @@ def hits_dict(out, n_hits=None):
 # reference-compatible entry points
 # ------------------------------------------------------------------------------------------
 
+def reference_rows(wfs, pids, neg_ids, n_neg):
+    """The reference's row layout of ``simulate_wfs`` (``sim_jax.py:716-721``): the sorted unique
+    main-pixel ids plus an appended -1, padded with -1 to ``pad_size(count, "unique_pixels", 0.2)``.
+
+    ``pids`` / ``wfs`` are the P rows with ids >= 0; ``neg_ids[:n_neg]`` the distinct negative main
+    pixel ids (``Simulator.negative_rows``).  The negative rows lead, in ascending order, then the
+    -1 rows; all of them hold zero waveforms (they carry no charge).  One host read (``n_neg``)."""
+    P = pids.shape[0]
+    k = int(n_neg.item()) if isinstance(n_neg, torch.Tensor) else int(n_neg)
+    tgt = pad_size(P + k + 1, "unique_pixels", 0.2)
+    front = tgt - P
+    ids = torch.cat([neg_ids[:k], torch.full((front - k,), -1, dtype=torch.int64, device=pids.device), pids])
+    wfs = torch.cat([torch.zeros((front, wfs.shape[1]), dtype=wfs.dtype, device=wfs.device), wfs])
+    return wfs, ids
+
+
 def simulate_wfs(params, response_template, tracks, fields, pad=True):
     """Waveforms of the pixels that are the main pixel of some segment (``sim_jax.py:689-736``).
 
     Returns ``(wfs, unique_pixels)``: ``wfs`` (P_pad, Nticks-1) float32 and ``unique_pixels``
-    (P_pad,) int64 sorted ascending.  With ``pad`` the row count is padded with ``-1`` ids (zero
-    waveforms) by the reference's ``pad_size`` history rule, so shapes match the reference.
+    (P_pad,) int64 sorted ascending.  With ``pad`` the rows are laid out as the reference's are
+    (``reference_rows``: negative ids first, then ``pad_size`` -1 rows, zero waveforms); without it
+    only the rows with ids >= 0 are returned.
     """
     sim = get_simulator(params, response_template)
     st = sim.forward(params, tracks, fields, want_wfs=True)
     wfs, pids = st.out["wfs"][: st.max_rows], st.out["pixel_ids"][: st.max_rows]
     if pad:
-        P = st.max_rows
-        tgt = pad_size(P + 1, "unique_pixels", 0.2)
-        extra = tgt - P
-        pids = torch.cat([torch.full((extra,), -1, dtype=torch.int64, device=pids.device), pids])
-        wfs = torch.cat([torch.zeros((extra, wfs.shape[1]), dtype=wfs.dtype, device=wfs.device), wfs])
+        neg, n_neg = sim.negative_rows(params, st.seg)
+        wfs, pids = reference_rows(wfs, pids, neg, n_neg)
     return wfs, pids
 
 

@@ -12,7 +12,7 @@ import torch
 
 from .autograd import LUTSimFunction, needs_grad, theta_of
 from .fee import adc2charge
-from .sim import get_simulator, hits_dict, infer_n_events
+from .sim import get_simulator, hits_dict, infer_n_events, reference_rows
 
 
 class SimulationStrategy:
@@ -29,6 +29,9 @@ class HipLUTSimulation(SimulationStrategy):
         self.response = response
 
     def predict(self, params, tracks, fields, rngkey=None):
+        """``unique_pixels`` and ``wfs`` have the reference's row layout (``sim.reference_rows``:
+        negative ids, then ``pad_size`` -1 rows with zero waveforms, then the pixel rows); the hit
+        arrays are compacted as ``parse_output`` does."""
         sim = get_simulator(params, self.response)
         n_events = infer_n_events(tracks, fields)
         if needs_grad(params):
@@ -38,14 +41,16 @@ class HipLUTSimulation(SimulationStrategy):
             adcs, z, wfs = LUTSimFunction.apply(theta, sim, base, tracks, fields, n_events)
             # the non-differentiable outputs were kept on the function's context
             o = _last_others(adcs)
+            wfs, ids = reference_rows(wfs, o["unique_pixels"], *o["negative_rows"])
             return dict(adcs=adcs, pixel_x=o["pixel_x"], pixel_y=o["pixel_y"], pixel_z=z, ticks=o["ticks"],
                         hit_prob=o["hit_prob"], event=o["event"], hit_pixels=o["hit_pixels"],
-                        unique_pixels=o["unique_pixels"], wfs=wfs)
+                        unique_pixels=ids, wfs=wfs)
         st = sim.forward(params, tracks, fields, n_events=n_events, want_wfs=True)
+        neg = sim.negative_rows(params, st.seg)
         h = hits_dict(st.out)
+        wfs, ids = reference_rows(st.out["wfs"][: st.max_rows], st.out["pixel_ids"][: st.max_rows], *neg)
         return dict(adcs=h["adcs"], pixel_x=h["pixel_x"], pixel_y=h["pixel_y"], pixel_z=h["pixel_z"], ticks=h["ticks"],
-                    hit_prob=h["hit_prob"], event=h["event"], hit_pixels=h["hit_pixels"],
-                    unique_pixels=st.out["pixel_ids"][: st.max_rows], wfs=st.out["wfs"][: st.max_rows])
+                    hit_prob=h["hit_prob"], event=h["event"], hit_pixels=h["hit_pixels"], unique_pixels=ids, wfs=wfs)
 
 
 def pad_to_closest_multiple(x, dims_to_pad=None, multiple=128, pad_value=0, pad_front=False):
