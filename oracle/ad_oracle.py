@@ -37,14 +37,28 @@ def vdrift_t(E, p):
 class Frozen:
     """Integer decisions and parameter-independent data of one event, from the NumPy oracle."""
 
-    def __init__(self, p, raw_lut, tracks, fields, tables=None):
+    def __init__(self, p, raw_lut, tracks, fields, tables=None, rows_subset_pids=None):
+        """With ``rows_subset_pids`` (pixel ids >= 0 that are rows of the event) only those rows
+        are built, replayed and returned, in ascending id order: the stripes landing elsewhere are
+        dropped, so large events stay cheap.  Their values equal the full chain's on those rows."""
         self.p = p
-        sim = O.simulate(p, raw_lut, tracks, fields, tables=tables)
+        self.tables = tables if tables is not None else O.build_templates(p, raw_lut)
+        tables = self.tables
+        subset = rows_subset_pids is not None
+        if not subset:
+            sim = O.simulate(p, raw_lut, tracks, fields, tables=tables)
+        else:
+            pids = np.unique(np.asarray(rows_subset_pids, np.int64))
+            wfs, U, q = O.simulate_wfs(p, raw_lut, tracks, fields, rows_subset_pids=pids, tables=tables)
+            r, found = O._rows(U, pids)
+            if not found.all() or (pids < 0).any():
+                raise ValueError("rows_subset_pids must be pixel ids >= 0 of rows of the event")
+            sim = O.simulate_stochastic(p, wfs[r], U[r])
+            sim.update(wfs=wfs[r], unique_pixels=U[r], q=q)
         self.out = sim
         q = sim["q"]
         self.q = q
         self.U = sim["unique_pixels"]
-        self.tables = tables if tables is not None else O.build_templates(p, raw_lut)
         f = lambda n: np.asarray(tracks[:, fields.index(n)], np.float64)  # noqa: E731
         self.z = f("z") - p.shift_z
         self.dE, self.dEdx = f("dE"), f("dEdx")
@@ -58,11 +72,17 @@ class Frozen:
         # stripes (sim_jax.py:152-154, 722-725)
         rb, fb = O._rows(self.U, q["pid"].reshape(N, -1))
         rn, fn = O._rows(self.U, q["pid_nb"])
-        rn = np.where(fn, rn, 0)
         self.bins = dict(seg=np.nonzero(fb)[0], k=np.nonzero(fb)[1], row=rb[fb],
                          cx=q["cx"].reshape(N, -1)[fb], cy=q["cy"].reshape(N, -1)[fb])
-        segn, kn = np.nonzero(np.ones_like(fn))
-        self.nbs = dict(seg=segn, row=rn.ravel(), cx=q["cx_nb"].ravel(), cy=q["cy_nb"].ravel())
+        if subset:
+            # neighbour stripes off the subset are dropped
+            segn, kn = np.nonzero(fn)
+            self.nbs = dict(seg=segn, row=rn[fn], cx=q["cx_nb"][fn], cy=q["cy_nb"][fn])
+        else:
+            # neighbours without a row land on row 0, the -1 row (sim_jax.py:723-725)
+            rn = np.where(fn, rn, 0)
+            segn, kn = np.nonzero(np.ones_like(fn))
+            self.nbs = dict(seg=segn, row=rn.ravel(), cx=q["cx_nb"].ravel(), cy=q["cy_nb"].ravel())
         self.dec = sim["grid"]["dec"]
         self.ticks = sim["grid"]["ticks"]
         self.valid_rows = self.U >= 0

@@ -270,6 +270,34 @@ def build_templates(p, raw, main_extent=None):
 # --------------------------------------------------------------------------------------
 
 
+def main_pixel_ids(p, tracks, fields, chunk=2_000_000):
+    """The main-pixel id of every segment alone (sim_jax.py:394-429: shift, drift's TPC choice,
+    bin pitches, pixel2id of the centre bin), chunked, for pixel sets of many-million-segment
+    batches where ``segment_quantities`` would materialise every bin.  Pinned by equality with
+    ``segment_quantities(...)["main_pid"]``."""
+    out = np.empty(len(tracks), np.int64)
+    b = _borders(p)
+    nsb = int(p.nb_sampling_bins_per_pixel)
+    w = p.pixel_pitch / nsb
+    m = p.size_margin
+    for lo in range(0, len(tracks), chunk):
+        t = tracks[lo:lo + chunk]
+        x = _col(t, fields, "x", np.float64) - p.shift_x
+        y = _col(t, fields, "y", np.float64) - p.shift_y
+        z = _col(t, fields, "z", np.float64) - p.shift_z
+        ev = np.asarray(t[:, fields.index("eventID")]).astype(np.int64)
+        zmin = np.minimum(b[:, 2, 1] - m, b[:, 2, 0] - m)
+        zmax = np.maximum(b[:, 2, 1] + m, b[:, 2, 0] + m)
+        cond = ((x[:, None] >= b[None, :, 0, 0] - m) & (x[:, None] <= b[None, :, 0, 1] + m)
+                & (y[:, None] >= b[None, :, 1, 0] - m) & (y[:, None] <= b[None, :, 1, 1] + m)
+                & (z[:, None] >= zmin[None]) & (z[:, None] <= zmax[None]))
+        plane = cond.astype(np.int64).argmax(axis=1)
+        bx = np.floor_divide(x - b[plane, 0, 0], w).astype(np.int64)
+        by = np.floor_divide(y - b[plane, 1, 0], w).astype(np.int64)
+        out[lo:lo + chunk] = pixel2id(p, np.floor_divide(bx, nsb), np.floor_divide(by, nsb), plane, ev)
+    return out
+
+
 def unique_pixels(q):
     """sim_jax.py:716-721 without the recompilation padding: sorted unique main pids with -1."""
     return np.unique(np.append(q["main_pid"].ravel(), -1)).astype(np.int64)

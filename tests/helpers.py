@@ -44,6 +44,70 @@ def make_tracks(rows, fields=None):
     return t, fields
 
 
+def mixed_event(params, n_c1=200):
+    """C1 segments, their mirror images in TPC 1, border and out-of-TPC segments, padding rows
+    (eventID -1 at the origin, as the reference's pad_batch leaves them)."""
+    from larndsim_amd import synthetic
+
+    tr, f = synthetic.workload("c1")
+    tr = tr[:n_c1].copy()
+    mir = tr[::3].copy()
+    for k in ("z", "z_start", "z_end"):
+        mir[:, f.index(k)] *= -1
+    b = np.asarray(params.tpc_borders)
+    extra, _ = make_tracks([
+        dict(x=b[0, 0, 0] + 0.03, y=b[0, 1, 0] + 0.25, z=14.0, dE=1.0, dEdx=2.1, dx=0.4),      # pixel-plane corner
+        dict(x=b[0, 0, 1] - 0.01, y=3.0, z=7.0, dE=1.0, dEdx=2.1, dx=0.4),                    # far x border
+        dict(x=b[0, 0, 0] - 0.015, y=1.0, z=9.0, dE=1.0, dEdx=2.1, dx=0.4),                   # inside the margin, off-plane
+        dict(x=2.0, y=2.0, z=31.5, dE=1.0, dEdx=2.1, dx=0.4),                                 # beyond the anode: no charge
+        dict(x=2.0, y=2.0, z=30.5, dE=1.0, dEdx=2.1, dx=0.4),                                 # next to the anode
+        dict(x=-3.0, y=4.0, z=0.17, dE=1.0, dEdx=2.1, dx=0.4),                                # next to the cathode
+        dict(x=80.0, y=2.0, z=10.0, dE=1.0, dEdx=2.1, dx=0.4),                                # off every TPC and plane
+    ], f)
+    pad = np.zeros((5, len(f)), np.float32)
+    pad[:, f.index("eventID")] = -1
+    return np.concatenate([tr, mir, extra, pad]), f
+
+
+def compare_hits(st, ref, tag=""):
+    """The compacted hit arrays of a forward state against the oracle's ``parse_output`` (order
+    (row, hit slot), sim_jax.py:619-647): ids, ticks and events bit-exact, hit_prob exact, ADC to
+    1e-5 relative (1e-4 counts absolute), pixel x, y and z to 1e-5 cm (float32 of a float64 value).
+    Rows whose hit ticks differ from the oracle's (threshold ties) are left out of the value
+    comparison; their share of slots is at most 1e-3.  Returns the number of hits compared."""
+    from larndsim_amd.sim import hits_dict
+
+    H = st.out["adc"].shape[1]
+    P = st.max_rows
+    U = ref["unique_pixels"]
+    valid = U >= 0
+    n_inv = int((~valid).sum())
+    assert valid[n_inv:].all(), "negative ids lead the oracle's rows"
+    g = ref["grid"]
+    ticks = st.out["ticks"][:P].cpu().numpy()
+    same = ticks == g["ticks"][valid].astype(np.int64)
+    assert same.mean() >= 1 - 1e-3, tag
+    rows_ok = same.all(axis=1)
+    # oracle hits: flat (row over U, slot) positions of its compaction mask, moved onto the P rows
+    mask = (g["hit_prob"] > 1e-5) & (ref["unique_pixels"][:, None] >= 0)
+    o_flat = np.nonzero(mask.ravel())[0] - n_inv * H
+    h = {k: v.cpu().numpy() for k, v in hits_dict(st.out).items()}
+    g_flat = h["hit_index"].astype(np.int64)
+    assert np.all(np.diff(g_flat) > 0), f"{tag}: hits not in (row, slot) order"
+    keep_g = rows_ok[g_flat // H]
+    keep_o = rows_ok[o_flat // H]
+    assert np.array_equal(g_flat[keep_g], o_flat[keep_o]), f"{tag}: compacted hit positions differ"
+    n = int(keep_g.sum())
+    for k in ("ticks", "event", "hit_pixels"):
+        assert np.array_equal(h[k][keep_g].astype(np.int64), ref[k][keep_o].astype(np.int64)), f"{tag}: {k}"
+    assert np.array_equal(h["hit_prob"][keep_g], ref["hit_prob"][keep_o].astype(np.float32)), f"{tag}: hit_prob"
+    np.testing.assert_allclose(h["adcs"][keep_g], ref["adcs"][keep_o], rtol=1e-5, atol=1e-4, err_msg=f"{tag}: adcs")
+    for k in ("pixel_x", "pixel_y", "pixel_z"):
+        np.testing.assert_allclose(h[k][keep_g], ref[k][keep_o], rtol=0, atol=1e-5, err_msg=f"{tag}: {k}")
+    print(f"[{tag}] {n} hits compared ({len(g_flat) - n} on tie rows)", flush=True)
+    return n
+
+
 def compare_forward(sim, params, lut, tracks, fields, tables=None, n_events=None, tag=""):
     """Run the HIP forward and the FP64 oracle on the same segments and check: pixel rows
     bit-exact, waveforms within 1e-5 of each row's peak, hit ticks exact on >= 99.9 % of slots

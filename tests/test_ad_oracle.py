@@ -90,3 +90,37 @@ def test_waveform_loss_central_differences(params4, lut, c1, tables):
     errs = [abs(fd("eField", hr) - g[i]) / abs(g[i]) for hr in (1e-3, 1e-6)]
     print("eField relative difference at h = 1e-3, 1e-6:", errs)
     assert errs[1] < 1e-3 and errs[1] < errs[0] / 5
+
+
+def test_rows_subset_equals_full_chain(params4, lut, c1, tables):
+    """Frozen over a subset of rows (for large events) returns the full chain's values on those
+    rows, and the gradient of a loss on them equals the full chain's gradient of the same loss."""
+    tr, f = c1
+    tr = tr[:300]
+    full = A.Frozen(params4, lut, tr, f, tables=tables)
+    U = full.U
+    rows = np.nonzero(U >= 0)[0]
+    rng = np.random.default_rng(3)
+    pick = np.sort(rng.choice(rows, size=12, replace=False))
+    sub = A.Frozen(params4, lut, tr, f, tables=tables, rows_subset_pids=U[pick])
+    assert np.array_equal(sub.U, U[pick])
+    assert np.array_equal(sub.ticks, full.ticks[pick])
+    th = full.theta0()
+    with torch.no_grad():
+        a_f, z_f, w_f = full.forward(th)
+        a_s, z_s, w_s = sub.forward(th)
+    scale = float(w_f.abs().max())
+    np.testing.assert_allclose(w_s.numpy(), w_f.numpy()[pick], rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(a_s.numpy(), a_f.numpy()[pick], rtol=1e-11, atol=1e-10)
+    np.testing.assert_allclose(z_s.numpy(), z_f.numpy()[pick], rtol=1e-13, atol=1e-13)
+    hits = sub.ticks < 1997
+    assert hits.sum() > 3
+    w_adc = rng.normal(size=hits.shape) * hits
+    w_z = rng.normal(size=hits.shape) * hits
+    g_s, _ = sub.vjp(w_adc, w_z)
+    W_adc = np.zeros(full.ticks.shape)
+    W_z = np.zeros(full.ticks.shape)
+    W_adc[pick], W_z[pick] = w_adc, w_z
+    vr = full.valid_rows
+    g_f, _ = full.vjp(W_adc[vr], W_z[vr])
+    np.testing.assert_allclose(g_s, g_f, rtol=1e-10, atol=1e-14)

@@ -58,32 +58,67 @@ def test_backward_matches_autograd(sim, params4, c1, frozen, seed):
     assert err.max() < GRAD_TOL
 
 
-@pytest.mark.parametrize("kw", [dict(number_pix_neighbors=0), dict(MAX_ADC_VALUES=4), dict(signal_length=187)],
-                         ids=["n0", "H4", "L187"])
-def test_backward_other_settings(params4, lut, c1, kw):
-    """number_pix_neighbors = 0 (rows get no neighbour stripes but still the diffusion bins that
-    spill over from segments of the adjacent pixels: forward and adjoint walk the 3x3 cells), fewer
-    self-trigger passes than the compiled maximum, and the longest window."""
+def _edge_event(n_c1=300):
+    """C1 segments plus segments next to the anode of each TPC, whose windows start at or before
+    output tick 0 (window start nt - L - k0 <= 0 once the drift time to the cathode exceeds
+    (nt - L) ticks), so the clamped first samples are exercised in both passes of the adjoint."""
+    from larndsim_amd import synthetic
+
+    tr, f = synthetic.workload("c1")
+    near = tr[n_c1:n_c1 + 120].copy()
+    z = near[:, f.index("z")]
+    near[:, f.index("z")] = 28.6 + np.mod(z, 1.9)
+    near[::2, f.index("z")] *= -1  # every other one at the TPC 1 anode
+    for k in ("z_start", "z_end"):
+        near[:, f.index(k)] = near[:, f.index("z")]
+    return np.concatenate([tr[:n_c1], near]), f
+
+
+VARIANTS = [dict(nb_tran_diff_bins=3), dict(nb_tran_diff_bins=1), dict(number_pix_neighbors=0),
+            dict(MAX_ADC_VALUES=4), dict(signal_length=187)]
+
+
+@pytest.mark.parametrize("dense", [False, True], ids=["hits", "waveforms"])
+@pytest.mark.parametrize("kw", VARIANTS, ids=["nb3", "nb1", "n0", "H4", "L187"])
+def test_backward_other_settings(params4, lut, kw, dense):
+    """The adjoint in every compiled bin-count instance (5 is the default; 3 and 1 merge bins
+    differently), with number_pix_neighbors = 0 (rows get no neighbour stripes but still the
+    diffusion bins that spill over from segments of the adjacent pixels), fewer self-trigger passes
+    than the compiled maximum, and the longest window; with cotangents on the hit grids only and
+    with a cotangent on every waveform tick as well (the dense variant), on an event whose
+    near-anode windows start at or before tick 0."""
     from larndsim_amd.sim import Simulator
     from oracle import ad_oracle as A
     from oracle import oracle as O
 
     p0 = params4.replace(**kw)
-    tr, f = c1
-    tr, f = tr[:400], f
+    tr, f = _edge_event()
     fz = A.Frozen(p0, lut, tr, f, tables=O.build_templates(p0, lut))
+    k0 = fz.k0[fz.q["mask"] > 0]
+    assert (1950 - int(p0.signal_length) - k0 <= 0).any()
     s0 = Simulator(p0, lut)
-    print(f"[gpu] backward c1[:400] {kw}", flush=True)
+    print(f"[gpu] backward edge event {kw} dense={dense}", flush=True)
     st = s0.forward(p0, tr, f, want_tape=True)
     P = st.max_rows
     assert np.array_equal(st.out["ticks"][:P].cpu().numpy(), fz.ticks[fz.valid_rows].astype(np.int64))
     w_adc, w_z = _cotangents(fz, 3)
-    g_ref, _ = fz.vjp(w_adc, w_z)
-    g = s0.backward(st, torch.from_numpy(w_adc.astype(np.float32)), torch.from_numpy(w_z.astype(np.float32))).cpu().numpy()
+    vr = torch.as_tensor(fz.valid_rows)
+    th = fz.theta0().requires_grad_(True)
+    adc, z, wfs = fz.forward(th)
+    loss = (torch.as_tensor(w_adc) * adc[vr]).sum() + (torch.as_tensor(w_z) * z[vr]).sum()
+    w_wfs = None
+    if dense:
+        w_wfs = np.random.default_rng(7).normal(size=(P, s0.nout))
+        loss = loss + (torch.as_tensor(w_wfs) * wfs[vr]).sum()
+    loss.backward()
+    g_ref = th.grad.detach().numpy()
+    g = s0.backward(st, torch.from_numpy(w_adc.astype(np.float32)), torch.from_numpy(w_z.astype(np.float32)),
+                    None if w_wfs is None else torch.from_numpy(w_wfs.astype(np.float32))).cpu().numpy()
     print("gpu ", g)
     print("ref ", g_ref)
     mask = np.abs(g_ref) > 0
     assert mask.sum() >= 5
+    assert g[4] == 0.0
     assert _rel(g[mask], g_ref[mask]).max() < GRAD_TOL
 
 

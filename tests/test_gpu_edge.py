@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import compare_forward, make_tracks
+from helpers import compare_forward, mixed_event
 
 pytestmark = pytest.mark.gpu
 
@@ -16,30 +16,6 @@ def sim(params4, lut):
     from larndsim_amd.sim import Simulator
 
     return Simulator(params4, lut)
-
-
-def _mixed(params, n_c1=200):
-    """C1 segments, their mirror images in TPC 1, border and out-of-TPC segments, padding."""
-    from larndsim_amd import synthetic
-
-    tr, f = synthetic.workload("c1")
-    tr = tr[:n_c1].copy()
-    mir = tr[::3].copy()
-    for k in ("z", "z_start", "z_end"):
-        mir[:, f.index(k)] *= -1
-    b = np.asarray(params.tpc_borders)
-    extra, _ = make_tracks([
-        dict(x=b[0, 0, 0] + 0.03, y=b[0, 1, 0] + 0.25, z=14.0, dE=1.0, dEdx=2.1, dx=0.4),      # pixel-plane corner
-        dict(x=b[0, 0, 1] - 0.01, y=3.0, z=7.0, dE=1.0, dEdx=2.1, dx=0.4),                    # far x border
-        dict(x=b[0, 0, 0] - 0.015, y=1.0, z=9.0, dE=1.0, dEdx=2.1, dx=0.4),                   # inside the margin, off-plane
-        dict(x=2.0, y=2.0, z=31.5, dE=1.0, dEdx=2.1, dx=0.4),                                 # beyond the anode: no charge
-        dict(x=2.0, y=2.0, z=30.5, dE=1.0, dEdx=2.1, dx=0.4),                                 # next to the anode
-        dict(x=-3.0, y=4.0, z=0.17, dE=1.0, dEdx=2.1, dx=0.4),                                # next to the cathode
-        dict(x=80.0, y=2.0, z=10.0, dE=1.0, dEdx=2.1, dx=0.4),                                # off every TPC and plane
-    ], f)
-    pad = np.zeros((5, len(f)), np.float32)
-    pad[:, f.index("eventID")] = -1
-    return np.concatenate([tr, mir, extra, pad]), f
 
 
 def test_empty(sim, params4):
@@ -52,7 +28,7 @@ def test_empty(sim, params4):
 
 
 def test_mixed_event(sim, params4, lut, tables):
-    tr, f = _mixed(params4)
+    tr, f = mixed_event(params4)
     compare_forward(sim, params4, lut, tr, f, tables=tables, n_events=1, tag="mixed")
 
 
@@ -61,7 +37,7 @@ def test_recombination_models(sim, params4, lut, tables, mode):
     from larndsim_amd.consts import RecombinationMode
 
     p = params4.replace(recombination_mode=RecombinationMode[mode])
-    tr, f = _mixed(params4, n_c1=300)
+    tr, f = mixed_event(params4, n_c1=300)
     compare_forward(sim, p, lut, tr, f, tables=tables, n_events=1, tag=mode)
 
 
@@ -72,7 +48,7 @@ def test_other_configurations(params4, lut, nn, nb):
 
     p = params4.replace(number_pix_neighbors=nn, nb_tran_diff_bins=nb)
     s = Simulator(p, lut)
-    tr, f = _mixed(p, n_c1=300)
+    tr, f = mixed_event(p, n_c1=300)
     compare_forward(s, p, lut, tr, f, tables=O.build_templates(p, lut), n_events=1, tag=f"n{nn}b{nb}")
 
 
@@ -86,5 +62,5 @@ def test_window_length_and_hit_slots(params4, lut, kw):
 
     p = params4.replace(**kw)
     s = Simulator(p, lut)
-    tr, f = _mixed(p, n_c1=300)
+    tr, f = mixed_event(p, n_c1=300)
     compare_forward(s, p, lut, tr, f, tables=O.build_templates(p, lut), n_events=1, tag=str(kw))

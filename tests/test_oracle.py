@@ -460,3 +460,20 @@ def test_mse_loss_closed_form():
     # pixel 7 in both, pixel 8 only in a, pixel 9 only in b
     v = O.mse_loss(a, np.array([7, 8]), b, np.array([7, 9]))
     assert v == (0 + 1) + (9 + 16) + (25 + 25)
+
+
+def test_main_pixel_ids_match_full_quantities(p1):
+    """The light main-pixel function (for multi-million-segment pixel sets) equals the centre bin
+    of segment_quantities on an event with both TPCs, borders, off-plane and padding rows, also
+    with shifted positions and in small chunks."""
+    import sys
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from helpers import mixed_event
+
+    tr, f = mixed_event(p1, n_c1=600)
+    for p in (p1, p1.replace(shift_x=0.1234, shift_y=-0.0777, shift_z=0.3125)):
+        want = O.segment_quantities(p, tr, f)["main_pid"]
+        assert (want < -1).any() and (want == -1).any() and (want >= 0).any()
+        assert np.array_equal(O.main_pixel_ids(p, tr, f), want)
+        assert np.array_equal(O.main_pixel_ids(p, tr, f, chunk=97), want)
