@@ -164,6 +164,17 @@ int lsim_fee_query(lsim_ctx *ctx, int64_t n_rows, size_t *ws_bytes);
 int lsim_fee_forward(lsim_ctx *ctx, const lsim_params *p, const float *wfs, const int64_t *pixel_ids, int64_t n_rows,
                      void *ws, size_t ws_bytes, const lsim_fwd_out *out, void *stream);
 
+/* lsim_fee_forward with no workspace argument and no allocation of any kind: its scratch (hit
+ * counts, offsets, scan partials) lives in a file-scope device array that each GPU holds until
+ * the process ends (about 8.4 MB).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  At most LSIM_NOALLOC_MAX_ROWS rows (LSIM_E_SPACE beyond).  Calls share their GPU's
+ * scratch, so calls that overlap on one GPU (other streams, other threads) race; calls ordered on
+ * one stream do not.  Results equal lsim_fee_forward's bit for bit. */
+#define LSIM_NOALLOC_MAX_ROWS (1 << 20)
+int lsim_fee_forward_noalloc(lsim_ctx *ctx, const lsim_params *p, const float *wfs, const int64_t *pixel_ids,
+                             int64_t n_rows, const lsim_fwd_out *out, void *stream);
+
 /* Copy the per-segment records of the last prepare call (in main-pixel order) for inspection:
  * rec_out[n_seg * 80 bytes] = {int k0, uxy; float n_e, f; int bx, by, idx; float a, b, c, wx[5], wy[5]}
  * (uxy = (bx mod 10) | (by mod 10) << 16),

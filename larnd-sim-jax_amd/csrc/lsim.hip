@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -649,6 +650,64 @@ extern "C" {
 int lsim_fee_forward(lsim_ctx* c, const lsim_params* p, const float* wfs, const int64_t* pixel_ids, int64_t n_rows,
                      void* ws, size_t ws_bytes, const lsim_fwd_out* out, void* stream) {
   return fee_forward_impl(c, p, wfs, pixel_ids, n_rows, ws, ws_bytes, out, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// lsim_fee_forward_noalloc: the workspace of lsim_fee_forward in a file-scope device array
+// ------------------------------------------------------------------------------------------
+namespace {
+
+// lsim_fee_query's layout for LSIM_NOALLOC_MAX_ROWS rows: hit counts and offsets (one int per
+// row each), the scan's tile sums and its total, each run 256-byte aligned.
+constexpr int64_t kNoAllocRows = LSIM_NOALLOC_MAX_ROWS;
+constexpr size_t kNoAllocBytes =
+    (size_t)2 * kNoAllocRows * 4 + (size_t)((kNoAllocRows + kScanTile - 1) / kScanTile + 4) * 4 + 16 + 5 * 256;
+
+// Each GPU the library runs on holds its own copy until the process ends.  Host code never
+// passes this name to a kernel or a copy: it is a host-side stand-in, and the device address
+// comes from hipGetSymbolAddress (scratch_on_current_device).
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(LSIM_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(LSIM_E_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return LSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsim_fee_forward_noalloc(lsim_ctx* c, const lsim_params* p, const float* wfs, const int64_t* pixel_ids,
+                             int64_t n_rows, const lsim_fwd_out* out, void* stream) {
+  if (!c) return fail(LSIM_E_ARG, "NULL argument");
+  if (n_rows > kNoAllocRows)
+    return fail(LSIM_E_SPACE, "n_rows " + std::to_string(n_rows) + " exceeds LSIM_NOALLOC_MAX_ROWS (" +
+                                  std::to_string(kNoAllocRows) + ")");
+  size_t need = 0;
+  int rc = lsim_fee_query(c, n_rows, &need);
+  if (rc) return rc;
+  if (need > kNoAllocBytes) return fail(LSIM_E_SPACE, "scratch array smaller than the workspace layout");
+  void* ws = nullptr;
+  rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  return fee_forward_impl(c, p, wfs, pixel_ids, n_rows, ws, kNoAllocBytes, out, (hipStream_t)stream);
 }
 
 int lsim_segment_records(lsim_ctx* c, int64_t n_seg, int32_t n_events, void* ws, size_t ws_bytes, void* rec_out,
