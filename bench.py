@@ -1,18 +1,20 @@
 """Benchmark of the LUT simulation chain on one MI355X.
 
-Headline workload: one event of 200k track segments (500 tracks of 10-150 cm in a 30x30x30 cm
-drift volume, module0 geometry, 45x45x1950 LUT, 4 neighbour pixels, 5x5 diffusion bins, noise
-off).  A step = one forward pass (quench -> drift -> pixel rows -> waveforms -> self-trigger ->
-hits) over device-resident segments.  Prints one JSON line.
+Headline workload: the batched forward of 128 events whose segment counts are log-uniform in
+[2k, 400k] (9.16 M segments in a 30x30x30 cm drift volume, module0 geometry, 45x45x1950 LUT, 4
+neighbour pixels, 5x5 diffusion bins, noise off).  A step = one hipGraph replay of the whole
+chain (quench -> drift -> pixel rows -> waveforms -> self-trigger -> hits) over device-resident
+segments.  Prints one JSON line.
 
-Also measured in the same run (extra keys): forward+backward (7-parameter gradient) on the same
-event, the plain-forward finite-difference gradient baseline (15 forwards), the batched forward
-of 128 uneven events in one hipGraph replay, the batched forward+backward of 32 uneven events in
-one hipGraph replay against the per-event forward-then-finite-difference gradient, a 200-step
-Adam fit of the 7 parameters on those 32 events (steps/s, parameter recovery) with the hit loss
-and a 100-step one with a loss on the waveforms, and the CPU
-baseline (the float32 NumPy restatement of the reference algorithm on a bounded sample, on the
-host cores).
+Also measured in the same run (extra keys): the 200k-segment event alone, forward+backward
+(7-parameter gradient) on it, the plain-forward finite-difference gradient baseline (15
+forwards), the batched forward+backward of 32 uneven events in one hipGraph replay against the
+per-event forward-then-finite-difference gradient, a 200-step Adam fit of the 7 parameters on
+those 32 events (steps/s, parameter recovery) with the hit loss and a 100-step one with a loss
+on the waveforms, the per-output parity of the 1000-segment event against the FP64 oracle (with
+the threshold-tie fraction), and the CPU baselines (the float32 NumPy restatement of the
+reference algorithm: whole events of the batch, one per process on every core; and one thread on
+a bounded sample).
 
 ``--sdtw-only`` runs only the soft-DTW leg (value and gradient between two hit ADC sequences).
 ``--prob-fee-only`` runs only the probabilistic readout on the 200k-segment event's rows.
@@ -87,7 +89,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(params, lut, tracks, fields, n_sample: int):
+def cpu_baseline(params, lut, tracks, fields, n_sample: int, what: str):
     """The float32 NumPy restatement of the reference algorithm (materialise + bincount), single
     thread, on the first n_sample segments; LUT template build excluded (as on the GPU)."""
     from threadpoolctl import threadpool_limits
@@ -105,9 +107,164 @@ def cpu_baseline(params, lut, tracks, fields, n_sample: int):
         O.simulate_stochastic(params, wfs, U)
         dt = time.perf_counter() - t0
     return dict(value=n_sample / dt, unit="segments/s", cores=1, kind="port",
-                sample=f"first {n_sample} segments of the 200k-segment event, float32 NumPy restatement "
+                sample=f"first {n_sample} segments of {what}, float32 NumPy restatement "
                        f"(materialised contributions + bincount), 1 thread, {dt:.1f} s",
                 host_cpu_count=os.cpu_count())
+
+
+def _cpu_event_worker(job):
+    """One whole event through the float32 NumPy restatement in a fresh single-threaded process."""
+    params_kw, lut, tracks, fields = job
+    from threadpoolctl import threadpool_limits
+
+    sys.path.insert(0, ROOT)
+    import numpy as np
+
+    from larndsim_amd import consts
+    from oracle import oracle as O
+
+    params = consts.default_params(**params_kw)
+
+    with threadpool_limits(limits=1):
+        tables = O.build_templates(params, lut)
+        t0 = time.perf_counter()
+        wfs, U, q = O.simulate_wfs(params, lut, tracks, fields, tables=tables, dtype=np.float32)
+        O.simulate_stochastic(params, wfs, U)
+        return time.perf_counter() - t0
+
+
+def cpu_baseline_events(params_kw, lut, tracks, fields, sizes, procs: int = 16, target: int = 10_000):
+    """The same restatement on every core: the `procs` events of the batch closest in size to
+    `target` segments, one whole event per process (events are independent, so this is the
+    reference's own parallel decomposition).  The processes run together; each builds its LUT
+    templates, then times its event; the rate is the segments over the slowest event's time."""
+    import multiprocessing as mp
+
+    import numpy as np
+
+    ev = tracks[:, fields.index("eventID")].astype(np.int64)
+    picks = np.argsort(np.abs(np.asarray(sizes) - target), kind="stable")[:procs]
+    jobs = []
+    for e in picks:
+        sub = tracks[ev == e].copy()
+        sub[:, fields.index("eventID")] = 0
+        jobs.append((params_kw, lut, sub, fields))
+    n = int(sum(len(j[2]) for j in jobs))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(len(jobs)) as pool:
+        t0 = time.perf_counter()
+        per = pool.map(_cpu_event_worker, jobs, chunksize=1)
+        wall = time.perf_counter() - t0
+    return dict(value=n / max(per), unit="segments/s", cores=len(jobs), kind="port",
+                sample=f"{len(jobs)} whole events of the 128-event batch ({n} segments, {min(len(j[2]) for j in jobs)}-"
+                       f"{max(len(j[2]) for j in jobs)} each), one event per single-threaded process, float32 NumPy "
+                       f"restatement (materialised contributions + bincount); slowest event {max(per):.1f} s "
+                       f"(call {wall:.1f} s with process start and template builds)",
+                host_cpu_count=os.cpu_count())
+
+
+def parity_c1(simr, params, lut):
+    """Forward outputs of the 1000-segment event against the FP64 oracle, per output: pixel ids and
+    hit ticks (with the threshold-tie fraction), waveforms (relative to each row's peak, and
+    elementwise where |ref| > 1e-3 of the row peak), ADC on rows whose ticks agree, hit x/y/z."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, ROOT)
+    from larndsim_amd import synthetic
+    from larndsim_amd.sim import hits_dict
+    from oracle import oracle as O
+
+    tr, f = synthetic.workload("c1")
+    st = simr.forward(params, tr, f, n_events=1)
+    torch.cuda.synchronize()
+    ref = O.simulate(params, lut, tr, f)
+    U = ref["unique_pixels"]
+    valid = U >= 0
+    P = st.max_rows
+    pids = st.out["pixel_ids"][:P].cpu().numpy()
+    w = st.out["wfs"][:P].cpu().numpy().astype(np.float64)
+    rw = ref["wfs"][valid]
+    peak = np.maximum(np.abs(rw).max(axis=1, keepdims=True), 1e-30)
+    big = np.abs(rw) > 1e-3 * peak
+    ticks = st.out["ticks"][:P].cpu().numpy()
+    rt = ref["grid"]["ticks"][valid].astype(np.int64)
+    same = ticks == rt
+    ok = same.all(axis=1)
+    adc = st.out["adc"][:P].cpu().numpy()
+    ra = ref["grid"]["adcs"][valid]
+    h = hits_dict(st.out)
+    nh = int(h["adcs"].numel())
+    coords = {}
+    if nh == len(ref["adcs"]) and same.all():
+        for k in ("pixel_x", "pixel_y", "pixel_z"):
+            coords[k] = float(np.abs(h[k].cpu().numpy() - ref[k]).max())
+    return dict(workload="1 event, 1000 segments (10 tracks x 40 cm)", reference="FP64 NumPy oracle",
+                pixel_ids_exact=bool(np.array_equal(pids, U[valid])), rows=P,
+                wfs_max_err_rel_to_row_peak=float((np.abs(w - rw) / peak).max()),
+                wfs_max_rel_err_above_milli_peak=float((np.abs(w - rw)[big] / np.abs(rw[big])).max()),
+                tick_slots=int(same.size), tick_mismatch=int((~same).sum()),
+                threshold_tie_fraction=float((~same).mean()),
+                adc_max_rel_err=float((np.abs(adc[ok] - ra[ok]) / np.maximum(np.abs(ra[ok]), 1e-6)).max()),
+                hits=nh, hits_oracle=int(len(ref["adcs"])), hit_coordinate_max_abs_err_cm=coords)
+
+
+def kernel_source_stamp() -> str:
+    """sha256 (first 16 hex) of the HIP sources: ties a recorded counter measurement to the kernels
+    it measured."""
+    import hashlib
+
+    d = os.path.join(ROOT, "larnd-sim-jax_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".h", ".hip")):
+            with open(os.path.join(d, name), "rb") as f:
+                h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "rows_fwd_traffic_128_events.json")
+
+
+def _counter_rows(d, kernel_sub):
+    """(dispatch id, counter value) of every dispatch of a kernel whose name contains kernel_sub,
+    from the rocprofv3 counter-collection CSVs under d (one counter per run, summed over its
+    instances per dispatch)."""
+    import csv
+    import glob
+
+    vals = {}
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                if kernel_sub in row["Kernel_Name"]:
+                    k = int(row["Dispatch_Id"])
+                    vals[k] = vals.get(k, 0.0) + float(row["Counter_Value"])
+    return [vals[k] for k in sorted(vals)]
+
+
+def record_traffic(d, kernel_ms):
+    """Turn rocprofv3 FETCH_SIZE / WRITE_SIZE passes (d/fetch, d/write for the waveform kernel of
+    the 128-event batch; d/copy_fetch, d/copy_write for `--copy-calibrate`) into HBM bytes per
+    launch, stamped with the kernel sources and the kernel time, into TRAFFIC_JSON."""
+    fetch = _counter_rows(os.path.join(d, "fetch"), "k_rows_fwd")[1:]  # the first, cold dispatch excluded
+    write = _counter_rows(os.path.join(d, "write"), "k_rows_fwd")[1:]
+    cf = _counter_rows(os.path.join(d, "copy_fetch"), "copyBuffer")
+    cw = _counter_rows(os.path.join(d, "copy_write"), "copyBuffer")
+    copy_bytes = 256 << 20
+    ff = sum(cf) / len(cf) * 1024 / copy_bytes
+    wf = sum(cw) / len(cw) * 1024 / copy_bytes
+    fk, wk = sum(fetch) / len(fetch), sum(write) / len(write)
+    rec = dict(kernel="k_rows_fwd<5,4>", workload="128 events, 9 160 058 segments, 457 979 pixel rows",
+               dispatches_used=f"{len(fetch)} warm dispatches of `bench.py --profile-batch` (the first excluded)",
+               fetch_size_kib=fk, write_size_kib=wk, copy_fetch_kib=sum(cf) / len(cf),
+               copy_write_kib=sum(cw) / len(cw), copy_bytes=copy_bytes, fetch_factor=ff, write_factor=wf,
+               fetch_bytes=fk * 1024 / ff, write_bytes=wk * 1024 / wf,
+               bytes_per_launch=fk * 1024 / ff + wk * 1024 / wf, kernel_ms=kernel_ms,
+               source_stamp=kernel_source_stamp())
+    with open(TRAFFIC_JSON, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    return rec
 
 
 def chop_leg(dev, steps, n_tracks=350_000, precision=0.1, cpu=True):
@@ -345,11 +502,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--waves", type=int, default=0, help="waves per row workgroup (0 = autotune)")
+    ap.add_argument("--row-cost", type=int, default=256,
+                    help="per-row cost (pairs) of the XCD band cuts when --waves is given (else autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-bwd", action="store_true")
     ap.add_argument("--no-batched-bwd", action="store_true", help="skip the batched forward+backward of 32 events")
     ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--cpu-procs", type=int, default=16, help="processes of the all-core CPU baseline")
     ap.add_argument("--fit-steps", type=int, default=200, help="Adam steps of the 32-event gradient fit (0: skip)")
     ap.add_argument("--fit-wfs-steps", type=int, default=100,
                     help="Adam steps of the same fit with a loss on the waveforms (0: skip)")
@@ -363,7 +522,16 @@ def main():
                     help="profiling: only the probabilistic readout on the 200k-segment event's rows")
     ap.add_argument("--batched-bwd-only", action="store_true",
                     help="profiling: only the 32-event batched forward+backward graph replays (no baseline)")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="profiling: only eager forwards of the 128-event batch (for counter passes)")
+    ap.add_argument("--record-traffic", metavar="DIR",
+                    help="write the waveform kernel's HBM traffic from rocprofv3 counter passes under DIR "
+                         "(fetch/, write/, copy_fetch/, copy_write/) into profiles/, stamped; needs --kernel-ms")
+    ap.add_argument("--kernel-ms", type=float, default=None)
     a = ap.parse_args()
+    if a.record_traffic:
+        print(json.dumps(record_traffic(a.record_traffic, a.kernel_ms)))
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events would shard across "
               "GPUs with no exchange in the forward and one 8-value all-reduce per gradient step)", file=sys.stderr)
@@ -384,7 +552,8 @@ def main():
     from larndsim_amd import consts, sim as S, synthetic
 
     dev = torch.device("cuda")
-    params = consts.default_params(number_pix_neighbors=4, RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0)
+    params_kw = dict(number_pix_neighbors=4, RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0)
+    params = consts.default_params(**params_kw)
     lut = synthetic.make_lut(0)
     if a.chop_only:
         print(json.dumps(chop_leg(dev, a.steps, cpu=not a.no_cpu)))
@@ -403,22 +572,75 @@ def main():
         st = S.Simulator(params, lut).forward(params, torch.from_numpy(tracks).to(dev), fields, n_events=1)
         print(json.dumps(prob_fee_leg(params, st.out["wfs"][: st.max_rows].contiguous(), a.steps, cpu=not a.no_cpu)))
         return
+    simr = S.Simulator(params, lut)
+    if a.profile_batch:
+        bt, bf = synthetic.workload("c3")
+        brun = S.Runner(simr, params, torch.from_numpy(bt).to(dev), bf, n_events=128, want_wfs=True)
+        if a.waves:
+            S.set_row_waves(a.waves)
+        for i in range(a.steps):
+            brun.forward()
+            torch.cuda.synchronize()
+            log(f"[bench] profile forward {i + 1}/{a.steps}")
+        return
+
+    # ---- headline: the batched forward of 128 uneven events, one graph replay per step ----
+    log("[bench] generating 128 events (log-uniform 2k-400k segments)")
+    bt, fields = synthetic.workload("c3")
+    sizes3 = synthetic.event_sizes("c3")
+    NB = bt.shape[0]
+    btr = torch.from_numpy(bt).to(dev)
+    brun = S.Runner(simr, params, btr, fields, n_events=128, want_wfs=True)
+    PB = brun.rows
+    log(f"[bench] N={NB} rows={PB}")
+    if a.waves:
+        S.set_row_waves(a.waves)
+        S.set_row_cost(a.row_cost)
+        waves, row_cost, tune = a.waves, a.row_cost, {}
+    else:
+        waves, row_cost, tune = S.autotune_rows(brun)
+    log(f"[bench] waves per row workgroup: {waves}, row cost {row_cost} pairs (tuning ms: {tune})")
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        brun.forward()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        brun.forward()
+    torch.cuda.synchronize()
+    for _ in range(a.warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt / a.steps * 1e3
+    seg_s = NB / (dt / a.steps)
+    # the waveform kernel's own time: HIP events around its launches on the launch stream, over
+    # eager forwards of the same batch right after the timed region
+    S.timing(True)
+    for _ in range(3):
+        brun.forward()
+    k_ms, k_n = S.timing_read()
+    S.timing(False)
+    k_avg = k_ms / max(k_n, 1)
+    bytes_fwd = algorithmic_bytes_fwd(NB, PB)
+    achieved = bytes_fwd / (k_avg * 1e-3) / 1e9
+    log(f"[bench] batched {NB} segments {ms_step:.2f} ms -> {seg_s:.3e} seg/s; waveform kernel {k_avg:.2f} ms")
+    del brun, btr, g
+
+    extra = {}
+    # ---- the 200k-segment event alone ----
     log("[bench] generating the 200k-segment event")
     tracks, fields = synthetic.workload("c2")
     N = tracks.shape[0]
-    simr = S.Simulator(params, lut)
     trk = torch.from_numpy(tracks).to(dev)
     run = S.Runner(simr, params, trk, fields, n_events=1, want_wfs=True, want_tape=True)
     P = run.rows
-    log(f"[bench] N={N} rows={P}")
-    if a.waves:
-        S.set_row_waves(a.waves)
-        waves, tune = a.waves, {}
-    else:
-        waves, tune = S.autotune_row_waves(run)
-    log(f"[bench] waves per row workgroup: {waves} (tuning ms: {tune})")
-
-    # ---- plain forward (the headline) ----
     for _ in range(a.warmup):
         run.forward()
     torch.cuda.synchronize()
@@ -427,17 +649,18 @@ def main():
     for _ in range(a.steps):
         run.forward()
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    k_ms, k_n = S.timing_read()
+    dt1 = time.perf_counter() - t0
+    k1_ms, k1_n = S.timing_read()
     S.timing(False)
-    ms_step = dt / a.steps * 1e3
-    seg_s = N / (dt / a.steps)
-    k_avg = k_ms / max(k_n, 1)
-    bytes_fwd = algorithmic_bytes_fwd(N, P)
-    achieved = bytes_fwd / (k_avg * 1e-3) / 1e9
-    log(f"[bench] forward {ms_step:.3f} ms/step, {seg_s:.3e} seg/s; waveform kernel {k_avg:.3f} ms")
-
-    extra = {}
+    k1 = k1_ms / max(k1_n, 1)
+    b1 = algorithmic_bytes_fwd(N, P)
+    extra["single_event"] = dict(workload="1 event, 200k segments (500 tracks, 10-150 cm)", segments=N, pixel_rows=P,
+                                 ms_per_step=dt1 / a.steps * 1e3, segments_per_s=N / (dt1 / a.steps),
+                                 kernel="k_rows_fwd", kernel_ms=k1, algorithmic_bytes=b1,
+                                 achieved_gbs=b1 / (k1 * 1e-3) / 1e9,
+                                 frac=b1 / (k1 * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    log(f"[bench] 200k-segment event {dt1 / a.steps * 1e3:.3f} ms/step, {N / (dt1 / a.steps):.3e} seg/s; "
+        f"waveform kernel {k1:.3f} ms")
     # ---- forward + backward ----
     if not a.no_bwd:
         rng = np.random.default_rng(7)
@@ -486,42 +709,6 @@ def main():
                                                    fd_speedup=dtfd / dtw)
         log(f"[bench] fwd+bwd with a waveform cotangent {dtw * 1e3:.3f} ms")
         del st_w, d_w
-
-    # ---- batched forward: 128 uneven events, one graph replay ----
-    if not a.no_batched:
-        log("[bench] generating 128 events (log-uniform 2k-400k segments)")
-        bt, bf = synthetic.workload("c3")
-        NB = bt.shape[0]
-        btr = torch.from_numpy(bt).to(dev)
-        del bt
-        brun = S.Runner(simr, params, btr, bf, n_events=128, want_wfs=True)
-        brun.forward()
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            brun.forward()
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
-        with torch.cuda.graph(g):
-            brun.forward()
-        torch.cuda.synchronize()
-        for _ in range(2):
-            g.replay()
-        torch.cuda.synchronize()
-        bsteps = max(3, a.steps // 4)
-        t0 = time.perf_counter()
-        for _ in range(bsteps):
-            g.replay()
-        torch.cuda.synchronize()
-        dtg = (time.perf_counter() - t0) / bsteps
-        bbt = algorithmic_bytes_fwd(NB, brun.rows)
-        extra["batched"] = dict(events=128, segments=NB, rows=brun.rows, ms_per_step=dtg * 1e3,
-                                segments_per_s=NB / dtg, graph=True, algorithmic_bytes=bbt,
-                                hbm_frac=bbt / dtg / 1e9 / HBM_PEAK_GBS)
-        log(f"[bench] batched {NB} segments {dtg * 1e3:.2f} ms -> {NB / dtg:.3e} seg/s")
-        del brun, btr, g
 
     # ---- batched forward + backward: 32 uneven events, one graph replay, vs per-event FD ----
     if not a.no_batched_bwd:
@@ -695,24 +882,39 @@ def main():
         log(f"[bench] waveform-loss gradient check: {check}")
         del gtr, tgt
 
+    log("[bench] parity of the 1000-segment event against the FP64 oracle")
+    extra["parity"] = parity_c1(simr, params, lut)
+    log(f"[bench] parity {extra['parity']}")
+
     cpu = None
     if not a.no_cpu:
-        log("[bench] CPU baseline")
-        cpu = cpu_baseline(params, lut, tracks, fields, a.cpu_sample)
-        log(f"[bench] CPU {cpu['value']:.1f} seg/s")
+        log("[bench] CPU baseline on every core (whole events of the batch)")
+        cpu = cpu_baseline_events(params_kw, lut, bt, fields, sizes3, procs=a.cpu_procs)
+        log(f"[bench] CPU {cpu['value']:.1f} seg/s on {cpu['cores']} processes")
+        log("[bench] CPU baseline, one thread")
+        extra["cpu_baseline_1thread"] = cpu_baseline(params, lut, tracks, fields, a.cpu_sample,
+                                                     "the 200k-segment event")
+        log(f"[bench] CPU {extra['cpu_baseline_1thread']['value']:.1f} seg/s on 1 thread")
 
     achievable = copy_gbs(dev)
     log(f"[bench] device copy {achievable:.0f} GB/s")
     oc = onchip_per_segment(int(params.number_pix_neighbors), int(params.nb_tran_diff_bins), int(params.signal_length))
-    onchip = dict(per_segment=oc, lds_bytes=oc["lds_bytes"] * N, lds_gbs=oc["lds_bytes"] * N / (k_avg * 1e-3) / 1e9,
-                  lds_peak_gbs=LDS_PEAK_GBS, lds_frac=oc["lds_bytes"] * N / (k_avg * 1e-3) / 1e9 / LDS_PEAK_GBS,
-                  note="the algorithm's scatter-adds and LUT gathers as LDS bytes over the waveform kernel's time")
+    onchip = dict(per_segment=oc, lds_bytes=oc["lds_bytes"] * NB, lds_gbs=oc["lds_bytes"] * NB / (k_avg * 1e-3) / 1e9,
+                  lds_peak_gbs=LDS_PEAK_GBS, lds_frac=oc["lds_bytes"] * NB / (k_avg * 1e-3) / 1e9 / LDS_PEAK_GBS,
+                  note="the algorithm's scatter-adds and LUT gathers counted as LDS bytes over the waveform "
+                       "kernel's time (an upper bound: the kernel merges each (1-f, f) split into one "
+                       "read-modify-write); measured LDS and issue counters are in profiles/")
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "rows_fwd_c2_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    # HBM traffic per launch from counter passes, valid only for the kernel sources they measured
+    traffic, stamp = None, None
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as f:
+            rec = json.load(f)
+        stamp = dict(source_stamp=rec.get("source_stamp"), kernel_ms=rec.get("kernel_ms"),
+                     current=kernel_source_stamp())
+        stamp["stale"] = stamp["source_stamp"] != stamp["current"]
+        if not stamp["stale"]:
+            traffic = rec.get("bytes_per_launch")
 
     line = {
         "metric": "segments/s fwd & fwd+bwd (grad of 7 det params), 1 GPU, % of HBM roofline",
@@ -727,12 +929,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (f64 index math)",
         "data": "synthetic (seeded tracks and 45x45x1950 response LUT)",
-        "config": {"workload": "1 event, 200k segments (500 tracks, 10-150 cm), module0 geometry, "
-                               "45x45x1950 LUT, 4 neighbour pixels, 5x5 diffusion bins, noise off",
-                   "segments": N, "pixel_rows": P, "row_waves": waves},
+        "config": {"workload": "batched forward: 128 events with log-uniform 2k-400k segments (seed 3), one "
+                               "hipGraph replay per step, module0 geometry, 45x45x1950 LUT, 4 neighbour pixels, "
+                               "5x5 diffusion bins, noise off",
+                   "segments": NB, "events": 128, "pixel_rows": PB, "row_waves": waves, "row_cost": row_cost,
+                   "tuning_ms": tune},
         "roofline": {"bound": "hbm", "kernel": "k_rows_fwd", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": bytes_fwd, "kernel_ms": k_avg, "achievable": achievable,
+                     "traffic_stamp": stamp, "algorithmic_bytes": bytes_fwd, "kernel_ms": k_avg,
+                     "achievable": achievable,
                      "forward_frac": bytes_fwd / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "onchip": onchip,
         "cpu_baseline": cpu,

@@ -192,6 +192,11 @@ int lsim_timing_read(double *total_ms, int64_t *launches);
 /* Tuning: waves per pixel-row workgroup of the waveform kernel (1, 2, 4 or 8; default 4). */
 int lsim_set_row_waves(int waves);
 
+/* Tuning: the fixed cost of one pixel row in units of segment-row pairs (default 256), used to cut
+ * the rows into per-XCD bands of equal estimated work for the persistent waveform kernel.  Only
+ * the schedule changes: results are the same for every value. */
+int lsim_set_row_cost(int pairs);
+
 /* Backward workspace bytes for n_seg segments and max_rows rows. */
 int lsim_backward_query(lsim_ctx *ctx, int64_t n_seg, int64_t max_rows, size_t *bwd_ws_bytes);
 

@@ -325,6 +325,7 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
   constexpr int TPER = kMaxOut / NT;
   __shared__ int s_pe[kMaxH], s_pg[kMaxH], s_pi[kMaxH];
   __shared__ double s_pa[kMaxH];
+  __shared__ double s_qmax[NT / kWave];
   const int t = threadIdx.x;
   const int nout = d.nout;
   const int lane = t & (kWave - 1), wid = t / kWave;
@@ -332,18 +333,58 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
   // ticks interleaved over the threads (o = t + NT u): a wave's lanes read consecutive doubles
   for (int k = 0; k < H; ++k) {
     int cand = 0x7fffffff;
+    double qmax = -1.0e300;  // max of Q[1 .. nout-1], the values a crossing needs at or above thr
 #pragma unroll
     for (int u = 0; u < TPER; ++u) {
       const int o = t + NT * u;
       if (o <= nout - 2) {
-        if (Q[o + 1] >= d.thr && Q[o] <= d.thr && o < cand) cand = o;
+        const double q1 = Q[o + 1];
+        if (q1 >= d.thr && Q[o] <= d.thr && o < cand) cand = o;
+        qmax = q1 > qmax ? q1 : qmax;
       }
     }
     cand = wave_min_i(cand);
-    if (lane == 0) sh_int[wid] = cand;
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+      const double v = __shfl_xor(qmax, m, kWave);
+      qmax = v > qmax ? v : qmax;
+    }
+    if (lane == 0) {
+      sh_int[wid] = cand;
+      s_qmax[wid] = qmax;
+    }
     __syncthreads();
     int istar = 0x7fffffff;
-    for (int w = 0; w < NT / kWave; ++w) istar = sh_int[w] < istar ? sh_int[w] : istar;
+    double bmax = -1.0e300;
+    for (int w = 0; w < NT / kWave; ++w) {
+      istar = sh_int[w] < istar ? sh_int[w] : istar;
+      bmax = s_qmax[w] > bmax ? s_qmax[w] : bmax;
+    }
+    // reset index of a pass without a crossing (i* = nout - 2)
+    int g_none = nout - 2 + 1 + d.interval + 1;
+    if (g_none >= nout) g_none = nout - 1;
+    if (bmax < d.thr && (k > 0 || Q[g_none] >= 0.0)) {
+      // Nothing reaches the threshold: no crossing in this pass, and none later either.  A reset
+      // Q <- max(Q - Q[g], 0) with Q[g] >= 0 never raises a value above max(Q, 0) < thr.  After
+      // the first pass Q >= 0 everywhere, so Q[g] >= 0; in the first pass (the raw cumulative
+      // charge may be negative) it is checked.  Every remaining pass ends at i* = nout-2 (no hit)
+      // with e and g at their clamps, exactly as running it would; Q is not read again (the tape
+      // follows Q0).
+      if (t == 0) {
+        for (int kk = k; kk < H; ++kk) {
+          int ee = nout - 2 + 1 + d.interval;
+          if (ee >= nout) ee = nout - 1;
+          int gg = nout - 2 + 1 + d.interval + 1;
+          if (gg >= nout) gg = nout - 1;
+          s_pe[kk] = ee;
+          s_pg[kk] = gg;
+          s_pi[kk] = nout - 2;
+          s_pa[kk] = Q[ee];
+        }
+      }
+      __syncthreads();
+      break;
+    }
     if (istar == 0x7fffffff) istar = nout - 2;
     int e = istar + 1 + d.interval;
     if (e >= nout) e = nout - 1;
@@ -806,15 +847,116 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
   __syncthreads();
 }
 
+// ------------------------------------------------------------------------------------------
+// work-balanced XCD bands of pixel rows for the persistent row kernels
+// ------------------------------------------------------------------------------------------
+//
+// Workgroup i of a launch runs on XCD i % kXcd, so each XCD is given one contiguous band of rows
+// (its records and tables stay in that XCD's L2).  Rows differ in cost (the segments in their
+// neighbourhood), and equal row counts per band leave some XCDs with up to 1.4x the mean work on
+// uneven batches.  The bands are therefore cut at equal shares of the estimated work: segments in
+// the row's (2e+1)^2 neighbourhood cells plus a fixed per-row cost (row set-up, sum, cumulative
+// charge and self-trigger) in the same units.
+
+// Estimated work of every row: one thread per row.  Rows P..max_rows-1 hold no pixel: their hit
+// count is zeroed here for the hit scan over max_rows (the row kernel never visits them).
+__global__ void k_row_work(Dev d, const int2* __restrict__ rng, const int* __restrict__ row_cell,
+                           const int* __restrict__ n_rows_dev, int64_t max_rows, int row_cost, int* __restrict__ work,
+                           int* __restrict__ hit_cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t P = *n_rows_dev < max_rows ? *n_rows_dev : max_rows;
+  if (r >= P) {
+    if (r < max_rows) hit_cnt[r] = 0;
+    return;
+  }
+  const int cell = row_cell[r];
+  const int px = cell % d.gx - d.halo;
+  const int py = (cell / d.gx) % d.gy - d.halo;
+  const int evp = cell / (d.gx * d.gy);
+  int n = row_cost;
+  for (int a = -d.ne; a <= d.ne; ++a) {
+    const int qx = px - a;
+    if (qx < -d.halo || qx >= d.nx + d.halo) continue;
+    for (int b = -d.ne; b <= d.ne; ++b) {
+      const int qy = py - b;
+      if (qy < -d.halo || qy >= d.ny + d.halo) continue;
+      const int2 rg = rng[((int64_t)evp * d.gy + (qy + d.halo)) * d.gx + (qx + d.halo)];
+      n += rg.y - rg.x;
+    }
+  }
+  work[r] = n;
+}
+
+// Band starts band[0..kXcd] (band[kXcd] = P) at equal shares of the total work; one workgroup of
+// kBandThreads threads.  The equal-work start of band x is the first row whose exclusive work
+// prefix reaches x/kXcd of the total (every row's work is >= 1, so the prefix is strictly
+// increasing).  Each band then holds at most `slots` rows (the workgroups per XCD of the row
+// kernel, kXcd * slots >= P): start_x is clamped to [max(start_{x-1}, P - (kXcd - x) slots),
+// start_{x-1} + slots], which keeps the starts non-decreasing and leaves room for the later bands.
+constexpr int kBandThreads = 1024;
+
+__global__ __launch_bounds__(kBandThreads) void k_bands(const int* __restrict__ work, const int* __restrict__ n_rows_dev,
+                                                      int64_t max_rows, int64_t slots, int* __restrict__ band) {
+  __shared__ int64_t part[kBandThreads];
+  __shared__ int64_t cut[kXcd + 1];
+  const int t = threadIdx.x;
+  const int64_t P = *n_rows_dev < max_rows ? *n_rows_dev : max_rows;
+  const int64_t per = (P + kBandThreads - 1) / kBandThreads;
+  const int64_t r0 = (int64_t)t * per < P ? (int64_t)t * per : P;
+  const int64_t r1 = r0 + per < P ? r0 + per : P;
+  int64_t s = 0;
+  for (int64_t r = r0; r < r1; ++r) s += work[r];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kBandThreads; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int64_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  const int64_t total = part[kBandThreads - 1];
+  const int64_t ex = part[t] - s;
+  for (int x = 1; x < kXcd; ++x) {
+    const int64_t target = total * x / kXcd;
+    if (ex < target && target <= ex + s) {
+      int64_t r = r0, e = ex;
+      while (e < target) e += work[r++];
+      cut[x] = r;
+    }
+    if (target == 0 && t == 0) cut[x] = 0;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int64_t prev = 0;
+    band[0] = 0;
+    for (int x = 1; x < kXcd; ++x) {
+      int64_t lo = P - (int64_t)(kXcd - x) * slots;
+      lo = lo > prev ? lo : prev;
+      const int64_t hi = prev + slots;
+      int64_t v = cut[x] < lo ? lo : cut[x];
+      v = v > hi ? hi : v;
+      band[x] = (int)v;
+      prev = v;
+    }
+    band[kXcd] = (int)P;
+  }
+}
+
 #ifndef LSIM_FAR_GROUP
 #define LSIM_FAR_GROUP 4
 #endif
 constexpr int kFarGroup = LSIM_FAR_GROUP;  // independent neighbour-only pairs in flight per wave
 
+// XCD-aware, work-balanced mapping of workgroups onto rows: workgroups are dispatched round-robin
+// over the XCDs, so workgroup i runs on XCD x = i % kXcd; it takes slot i / kXcd of band x (k_bands:
+// contiguous rows cut at equal shares of the estimated work, so a band's records and tables stay
+// in that XCD's L2 and every XCD finishes at about the same time).  The grid holds kXcd times the
+// longest band allowed (rows_grid); the workgroups past their band's end exit at once.
 template <int NB, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const int2* __restrict__ rng, const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
+                                                            const int* __restrict__ band,
                                                             const float* __restrict__ vc, const float* __restrict__ dtot,
                                                             const PairRec* __restrict__ prs, int64_t nseg, RowOut out) {
   constexpr int NT = WAVES * kWave;
@@ -825,20 +967,11 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
   __shared__ int ncls[3];  // far cells, + the centre cell, + the 8 side cells
 
-  // XCD-aware mapping: workgroups are dispatched round-robin over the XCDs, so workgroup i runs
-  // on XCD i % kXcd.  Workgroups 0..P-1 map one-to-one onto rows 0..P-1: XCD x receives the
-  // q + (x < rem) workgroups with i % kXcd == x (q = P / kXcd, rem = P % kXcd) and gives them
-  // the contiguous band of rows starting at x*q + min(x, rem), so the records and tables its
-  // rows share stay in that XCD's L2.
   const int P0 = *n_rows_dev;
   const int P = P0 < max_rows ? P0 : (int)max_rows;
-  if ((int64_t)blockIdx.x >= P) {
-    if (blockIdx.x < max_rows && threadIdx.x == 0) out.hit_cnt[blockIdx.x] = 0;
-    return;
-  }
   const int xcd = (int)(blockIdx.x % kXcd);
-  const int q8 = P / kXcd, rem = P % kXcd;
-  const int r = xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
+  const int r = band[xcd] + (int)(blockIdx.x / kXcd);
+  if (r >= band[xcd + 1] || r >= P) return;  // past the band (the bound on P is a guard only)
   const int nout = d.nout;
   float* acc = (float*)smem;
   const int t = threadIdx.x;

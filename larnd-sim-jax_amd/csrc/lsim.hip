@@ -181,6 +181,8 @@ struct Ws {
   int* hit_off;
   int* scan_tmp;
   int* n_hits32;
+  int* row_work;  // [n] estimated work of each pixel row (XCD band cuts of the row kernel)
+  int* band;      // [kXcd + 1] first row of each XCD's band
   float* vc;    // [n][kWinPad] centre-pixel windows of each segment (sorted order, padded)
   float* dtot;  // [(2 nn + 1)^2][n] boundary corrections of every (segment, row offset) pair
   size_t bytes;
@@ -206,6 +208,8 @@ Ws layout(void* base, int64_t n, int64_t n_cells) {
   w.hit_off = c.take<int>(n);
   w.scan_tmp = c.take<int>(scan_tiles(n) + 4);
   w.n_hits32 = c.take<int>(4);
+  w.row_work = c.take<int>(n);
+  w.band = c.take<int>(kXcd + 1);
   w.vc = nullptr;
   w.dtot = nullptr;
   w.bytes = c.off + 256;
@@ -239,12 +243,30 @@ void sorted_buffers(const Ws& w, const Dev& d, uint32_t*& keys, uint32_t*& vals)
   vals = (passes & 1) ? w.v1 : w.v0;
 }
 
+// Fixed cost of one pixel row (set-up, row sum, cumulative charge, self-trigger) in units of
+// segment-row pairs, for the work estimate the XCD bands are cut on (lsim_set_row_cost).
+int g_row_cost = 256;
+
+// Rows each XCD's band may hold: 1.5 times an equal share of max_rows (the grid of the row kernel
+// is kXcd times this, so at most a third of its workgroups find no row).
+int64_t band_slots(int64_t max_rows) { return (3 * max_rows + 2 * kXcd - 1) / (2 * kXcd); }
+
+// Work estimate of every row and the XCD band cuts (k_row_work, k_bands) before the row kernel;
+// also zeroes the hit counts of the rows past P.
+void launch_row_bands(const Dev& d, const Ws& w, int64_t max_rows, hipStream_t st) {
+  hipLaunchKernelGGL(k_row_work, dim3((unsigned)((max_rows + 255) / 256)), dim3(256), 0, st, d, w.rng, w.row_cell,
+                     w.n_rows, max_rows, g_row_cost, w.row_work, w.hit_cnt);
+  hipLaunchKernelGGL(k_bands, dim3(1), dim3(kBandThreads), 0, st, w.row_work, w.n_rows, max_rows, band_slots(max_rows),
+                     w.band);
+}
+
 template <int NB, int WAVES>
 void launch_rows_nb(const Dev& d, const Tables& tb, const Ws& w, int64_t n_seg, int64_t max_rows, const RowOut& ro,
                     hipStream_t st) {
-  hipLaunchKernelGGL((k_rows_fwd<NB, WAVES>), dim3((unsigned)max_rows), dim3(WAVES * kWave),
-                     rows_lds_bytes(WAVES, d.nout), st, d, tb, w.rec, w.rng, w.row_cell, w.n_rows, max_rows, w.vc, w.dtot,
-                     w.prs, n_seg, ro);
+  const size_t lds = rows_lds_bytes(WAVES, d.nout);
+  const int64_t grid = kXcd * band_slots(max_rows);
+  hipLaunchKernelGGL((k_rows_fwd<NB, WAVES>), dim3((unsigned)grid), dim3(WAVES * kWave), lds, st, d, tb, w.rec, w.rng,
+                     w.row_cell, w.n_rows, max_rows, w.band, w.vc, w.dtot, w.prs, n_seg, ro);
 }
 
 template <int WAVES>
@@ -307,6 +329,12 @@ size_t lsim_tape_bytes(void) { return sizeof(FeeTape); }
 int lsim_set_row_waves(int waves) {
   if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return fail(LSIM_E_ARG, "waves must be 1, 2, 4 or 8");
   g_row_waves = waves;
+  return LSIM_OK;
+}
+
+int lsim_set_row_cost(int pairs) {
+  if (pairs < 1 || pairs > (1 << 20)) return fail(LSIM_E_ARG, "row cost must be in [1, 2^20] pairs");
+  g_row_cost = pairs;
   return LSIM_OK;
 }
 
@@ -492,6 +520,7 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_s
     }
   }
   if (max_rows > 0) {
+    launch_row_bands(d, w, max_rows, st);
     hipEvent_t stop_ev;
     timer_begin(st, &stop_ev);
     switch (g_row_waves) {

@@ -18,7 +18,7 @@ MAX_ADC = 16
 EXPORTS = (
     "lsim_last_error", "lsim_version", "lsim_tape_bytes", "lsim_ctx_create", "lsim_ctx_destroy", "lsim_lut_set",
     "lsim_forward_query", "lsim_forward_prepare", "lsim_forward_rows", "lsim_forward", "lsim_fee_query",
-    "lsim_fee_forward", "lsim_fee_forward_noalloc", "lsim_negative_rows_query", "lsim_negative_rows", "lsim_backward_query", "lsim_backward", "lsim_backward_wfs", "lsim_set_row_waves", "lsim_segment_records", "lsim_timing",
+    "lsim_fee_forward", "lsim_fee_forward_noalloc", "lsim_negative_rows_query", "lsim_negative_rows", "lsim_backward_query", "lsim_backward", "lsim_backward_wfs", "lsim_set_row_waves", "lsim_set_row_cost", "lsim_segment_records", "lsim_timing",
     "lsim_timing_read", "lsim_rbf_sums", "lsim_nearest", "lsim_chop_query", "lsim_chop_count", "lsim_chop_fill",
     "lsim_prob_fee_query", "lsim_prob_fee", "lsim_sdtw_query", "lsim_sdtw",
 )
@@ -108,6 +108,7 @@ def load(path: str = LIB_PATH):
         "lsim_backward_wfs": [vp, C.POINTER(ParamsC), C.POINTER(Segments), i32, vp, sz, i64, C.POINTER(FwdOut), vp,
                               vp, vp, vp, sz, vp, vp],
         "lsim_set_row_waves": [C.c_int],
+        "lsim_set_row_cost": [C.c_int],
         "lsim_segment_records": [vp, i64, i32, vp, sz, vp, vp, vp, vp],
         "lsim_timing": [C.c_int],
         "lsim_timing_read": [C.POINTER(C.c_double), C.POINTER(i64)],

@@ -493,21 +493,46 @@ def set_row_waves(w: int):
     _lib.check(_lib.load().lsim_set_row_waves(int(w)))
 
 
+def set_row_cost(pairs: int):
+    _lib.check(_lib.load().lsim_set_row_cost(int(pairs)))
+
+
+def _kernel_ms(runner: Runner, reps: int) -> float:
+    runner.forward()
+    torch.cuda.synchronize()
+    timing(True)
+    for _ in range(reps):
+        runner.forward()
+    ms, n = timing_read()
+    timing(False)
+    return ms / max(n, 1)
+
+
 def autotune_row_waves(runner: Runner, candidates=(1, 2, 4, 8), reps=3):
     """Pick the waves-per-row-workgroup setting with the shortest waveform-kernel time."""
     best, best_ms = None, None
     res = {}
     for w in candidates:
         set_row_waves(w)
-        runner.forward()
-        torch.cuda.synchronize()
-        timing(True)
-        for _ in range(reps):
-            runner.forward()
-        ms, n = timing_read()
-        timing(False)
-        res[w] = ms / max(n, 1)
+        res[w] = _kernel_ms(runner, reps)
         if best_ms is None or res[w] < best_ms:
             best, best_ms = w, res[w]
     set_row_waves(best)
     return best, res
+
+
+def autotune_rows(runner: Runner, waves=(1, 2, 4, 8), costs=(64, 256, 1024), reps=3):
+    """Tune the waveform kernel on the runner's workload: waves per row workgroup first (at the
+    default row cost), then the per-row cost the XCD bands are balanced with.  Both only change
+    the schedule and the per-wave partial sums; the settings stay in effect.  Returns
+    (waves, row cost, {setting: kernel ms})."""
+    set_row_cost(256)
+    w, res_w = autotune_row_waves(runner, waves, reps)
+    best, best_ms, res_c = None, None, {}
+    for c in costs:
+        set_row_cost(c)
+        res_c[c] = _kernel_ms(runner, reps)
+        if best_ms is None or res_c[c] < best_ms:
+            best, best_ms = c, res_c[c]
+    set_row_cost(best)
+    return w, best, {**{f"waves={k}": v for k, v in res_w.items()}, **{f"row_cost={k}": v for k, v in res_c.items()}}

@@ -229,7 +229,9 @@ def test_autograd_through_waveforms(params4, lut, c1):
     sim = Simulator(params4, lut)
     st = sim.forward(params4, tr, f, want_tape=True)
     d_adc = torch.zeros(st.max_rows * sim.H)
-    g2 = sim.backward(st, d_adc, None, w).cpu().numpy()
+    # predict's rows: the reference's leading negative-id and -1 rows (no charge), then the P pixel rows
+    assert (out["unique_pixels"][: out["wfs"].shape[0] - st.max_rows] < 0).all()
+    g2 = sim.backward(st, d_adc, None, w[w.shape[0] - st.max_rows:]).cpu().numpy()
     print("autograd", g, "direct", g2, flush=True)
     assert np.allclose(g, g2, rtol=1e-6, atol=0)
     assert np.abs(g).max() > 0

@@ -263,3 +263,19 @@ def test_new_entry_points_check_arguments(lib):
     assert lib.lsim_nearest(None, 5, None, None, None, 5, 6, None, None, None) < 0  # n_origin > nx
     assert lib.lsim_nearest(None, 0, None, None, None, 0, 0, None, None, None) == 0  # nothing to do
     assert lib.lsim_chop_query(0, C.byref(sz)) == 0
+
+
+def test_needs_grad_follows_grad_mode():
+    """Parameters that require gradients are recorded only in grad mode (under torch.no_grad()
+    predict takes the plain path, as the reference's predict runs in both modes)."""
+    import torch
+
+    from larndsim_amd import consts
+    from larndsim_amd.autograd import needs_grad
+
+    p = consts.default_params(RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0)
+    assert not needs_grad(p)
+    pg = p.replace(Ab=torch.tensor(float(p.Ab), dtype=torch.float64, requires_grad=True))
+    assert needs_grad(pg)
+    with torch.no_grad():
+        assert not needs_grad(pg)
