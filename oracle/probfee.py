@@ -76,6 +76,14 @@ def find_one_hit_step(q_sum, prev_c, prev_lp, sigma, theta, interval, V):
 def prob_fee(wfs, t_sampling, V, S, interval, theta, sigma, stop=1e-9):
     """fee_jax.py:386-461: (log_prob, charge) of shape (P, S, T - 1), plus the selection scores
     (P, S, T - 1) of every executed step (for the tests' tie checks)."""
+    out_lp, out_q, sels, _, _ = prob_fee_tape(wfs, t_sampling, V, S, interval, theta, sigma, stop)
+    return out_lp, out_q, sels
+
+
+def prob_fee_tape(wfs, t_sampling, V, S, interval, theta, sigma, stop=1e-9):
+    """prob_fee, also returning the selected ticks of every executed step, (P, S, V) (the discrete
+    decisions reverse-mode AD holds fixed: top_k is under stop_gradient, fee_jax.py:377), and the
+    number of executed steps."""
     wfs = np.asarray(wfs, dtype=np.float32)
     P, T = wfs.shape
     q_sum = np.cumsum((wfs * np.float32(t_sampling)).astype(np.float64), axis=1)
@@ -85,15 +93,85 @@ def prob_fee(wfs, t_sampling, V, S, interval, theta, sigma, stop=1e-9):
     out_lp = np.full((P, S, T - 1), -1000.0)
     out_q = np.zeros((P, S, T - 1))
     sels = np.full((P, S, T - 1), -np.inf)
+    tops = np.zeros((P, S, V), np.int64)
     active = True
+    n_steps = 0
     for s in range(S):
         if not active:
             continue
+        n_steps += 1
         totals = np.empty(P)
         for p in range(P):
             nc, nl, hit, esp, sel = find_one_hit_step(q_sum[p], c[p], lp[p], sigma, theta, interval, V)
+            tops[p, s] = np.argsort(-sel, kind="stable")[:V]
             c[p], lp[p] = nc, nl
             out_lp[p, s], out_q[p, s], sels[p, s] = hit, esp, sel
             totals[p] = logsumexp(nl, 0)
         active = bool(np.any(totals > np.log(stop)))
-    return out_lp, out_q, sels
+    return out_lp, out_q, sels, tops, n_steps
+
+
+def prob_fee_torch(wfs, t_sampling, V, S, interval, theta, sigma, tops, n_steps, q_f32=True):
+    """The executed steps of the beam search in torch float64 with the path selections ``tops``
+    held fixed, differentiable in ``wfs`` (P, T): (log_prob, charge), each (P, S, T - 1), steps past
+    ``n_steps`` at -1000 / 0 as the reference's inactive branch writes them.  The same formulas as
+    ``find_one_hit_step``, vectorised over rows and paths; torch.cummax sends the gradient of a
+    running maximum to the last index that attains it, torch.minimum splits ties evenly.  With
+    ``q_f32`` the charge per tick is the float32 product wfs * t_sampling, as in ``prob_fee_tape``
+    (the cast passes gradients unchanged); without it the whole function is float64 (for finite
+    differences)."""
+    import torch
+
+    P, T = wfs.shape
+    dev = wfs.device
+    zs = 1.0 / sigma
+    sh = torch.clamp(torch.arange(T - 1, device=dev) + interval + 1, max=T - 1)
+
+    def softplus(x):
+        return torch.logaddexp(x, torch.zeros((), dtype=x.dtype, device=dev))
+
+    def soft_max_t(x, lo, s):
+        return lo + softplus((x - lo) * s) / s
+
+    def soft_where_t(c, t, f, s):
+        w = torch.sigmoid(c * s)
+        return w * t + (1.0 - w) * f
+
+    def log_diff(a, b):
+        la, lb = torch.special.log_ndtr(a), torch.special.log_ndtr(b)
+        sd = torch.where(a > b, lb - la, torch.full_like(a, -1.0))
+        ne = soft_max_t(-torch.expm1(sd), 1e-30, 1e10)
+        lt = soft_max_t(torch.log(ne), -100.0, 10.0)
+        return soft_where_t(a - b, la + lt, -1000.0, 1000.0)
+
+    if q_f32:
+        q = (wfs.to(torch.float32) * np.float32(t_sampling)).to(wfs.dtype)
+    else:
+        q = wfs * t_sampling
+    q_sum = torch.cumsum(q, dim=1)
+    c = torch.zeros((P, V), dtype=wfs.dtype, device=dev)
+    lp = torch.full((P, V), -1000.0, dtype=wfs.dtype, device=dev)
+    lp = torch.cat([torch.zeros((P, 1), dtype=wfs.dtype, device=dev), lp[:, 1:]], dim=1)
+    rows = torch.arange(P, device=dev)[:, None]
+    tops = torch.as_tensor(tops, device=dev)
+    out_lp, out_q = [], []
+    esp = q_sum[:, sh] + theta - 0.5 * (q_sum[:, 1:] + q_sum[:, :-1])
+    for s in range(S):
+        if s >= n_steps:
+            out_lp.append(torch.full((P, T - 1), -1000.0, dtype=wfs.dtype, device=dev))
+            out_q.append(torch.zeros((P, T - 1), dtype=wfs.dtype, device=dev))
+            continue
+        qsl = q_sum[:, None, :] - c[:, :, None]
+        qmf, _ = torch.cummax(qsl, dim=2)
+        lg = log_diff((qmf[..., 1:] - theta) * zs, (qmf[..., :-1] - theta) * zs)
+        le = log_diff((qsl[..., sh] - theta) * zs, (qsl[..., :-1] - theta) * zs)
+        le = torch.minimum(le, lg)
+        le = soft_max_t(le, -1000.0, 1.0)
+        le = soft_where_t(esp[:, None, :] - theta, le, -1000.0, 10.0)
+        out_lp.append(torch.logsumexp(le + lp[:, :, None], dim=1))
+        out_q.append(esp)
+        tot = torch.logsumexp(lg + lp[:, :, None], dim=1)
+        t_k = tops[:, s]
+        lp = tot[rows, t_k]
+        c = q_sum[rows, torch.clamp(sh[t_k] + 1, max=T - 1)]
+    return torch.stack(out_lp, dim=1), torch.stack(out_q, dim=1)
