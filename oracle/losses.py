@@ -62,3 +62,22 @@ def nll_loss(Q, x, y, z, hit_prob, event, ref_Q, ref_x, ref_y, ref_z, ref_hit_pr
     lam = (p[:, None] * K).sum(dim=0)
     term1 = (torch.log(lam + 1e-10) * (rp > 0.5)).sum()
     return p.sum() - term1
+
+
+def mse_adc(Q, x, y, z, hit_prob, event, ref_Q, ref_x, ref_y, ref_z, ref_hit_prob, ref_event, sigma=1.0, lambda_Q=1.0,
+            dtype=torch.float64):
+    """losses_jax.py:13-39, 58-80 literally: the hit clouds (x + event 1e5, y, z) weighted by
+    Q hit_prob, their RBF-kernel MMD over every pair (no event blocking), plus the squared relative
+    error of the total weighted charge."""
+    w_ref = (ref_Q * ref_hit_prob).to(dtype)
+    w = (Q * hit_prob).to(dtype)
+    a = torch.stack((x.to(dtype) + event.to(dtype) * 1e5, y.to(dtype), z.to(dtype)), dim=-1)
+    b = torch.stack((ref_x.to(dtype) + ref_event.to(dtype) * 1e5, ref_y.to(dtype), ref_z.to(dtype)), dim=-1)
+
+    def K(u, v):
+        return torch.exp(-((u[:, None, :] - v[None, :, :]) ** 2).sum(dim=-1) / (2 * sigma ** 2))
+
+    sw, sr = w.sum(), w_ref.sum()
+    mmd = ((K(a, a) * w[:, None] * w[None, :]).sum() / sw ** 2 + (K(b, b) * w_ref[:, None] * w_ref[None, :]).sum() / sr ** 2
+           - 2 * (K(a, b) * w[:, None] * w_ref[None, :]).sum() / (sw * sr))
+    return mmd + lambda_Q * ((sw - sr) / (sr + 1e-6)) ** 2

@@ -117,3 +117,20 @@ def test_mse_time_closed_form():
     L, _ = mse_time(None, adcs, torch.tensor([5, 9]), ticks, ref, torch.tensor([5, 7]), ticks_ref)
     # pixel 5: (10 - 12)^2 + (0 - 40)^2; pixel 7: 0 (masked); pixel 9: 20^2 + 30^2
     assert float(L) == pytest.approx(4 + 1600 + 400 + 900)
+
+
+def test_dense_mse_adc_closed_forms():
+    """The dense mse_adc restatement: identical clouds give 0; one point each at distance d gives
+    MMD 2 - 2 exp(-d^2 / 2 sigma^2) with equal charges, plus the relative charge error squared."""
+    import math
+
+    from oracle import losses as OL
+
+    t = lambda *v: torch.tensor(v, dtype=torch.float64)  # noqa: E731
+    x, y, z, Q, hp, ev = t(1.0, 2.0), t(0.5, -1.0), t(3.0, 3.5), t(10.0, 20.0), t(1.0, 1.0), t(0.0, 1.0)
+    assert float(OL.mse_adc(Q, x, y, z, hp, ev, Q, x, y, z, hp, ev)) == pytest.approx(0.0, abs=1e-15)
+    d = 0.7
+    v = OL.mse_adc(t(5.0), t(0.0), t(0.0), t(0.0), t(1.0), t(0.0), t(5.0), t(d), t(0.0), t(0.0), t(1.0), t(0.0))
+    assert float(v) == pytest.approx(2 - 2 * math.exp(-d * d / 2), rel=1e-12)
+    v = OL.mse_adc(t(6.0), t(0.0), t(0.0), t(0.0), t(1.0), t(0.0), t(5.0), t(0.0), t(0.0), t(0.0), t(1.0), t(0.0))
+    assert float(v) == pytest.approx((1.0 / (5.0 + 1e-6)) ** 2, rel=1e-9)
