@@ -230,6 +230,23 @@ int lsim_backward_wfs(lsim_ctx *ctx, const lsim_params *p, const lsim_segments *
 int lsim_rbf_sums(const float *x, int64_t nx, const int32_t *lo, const int32_t *hi, const float *y, const float *c,
                   int64_t ny, float sigma, double *out, void *stream);
 
+/* The same sums over the whole event of each x_i, walking only the cells of a uniform grid that can
+ * hold a non-zero float32 kernel value (the reference forms the full kernel matrix, losses_jax.py:13-39;
+ * the pairs left out are those whose exp(-d^2 / (2 sigma^2)) is exactly 0 in float32, so the sums are
+ * those of every pair of the event, in another order).
+ * grid [9] float32 device array: lo (x, y, z), cell size h (x, y, z), cell counts G (x, y, z) as floats;
+ * the cell of a point is clamp(floor((v - lo) / h), 0, G - 1) per axis.  y and c [ny][4] sorted by
+ * y_key [ny] int64 = ((event * Gx + cx) * Gy + cy) * Gz + cz, ascending.  x [nx][4] with x_event [nx]
+ * int32, any order (sorting x by the same key keeps each wave's points close).  dims 3 (the fourth
+ * coordinate is ignored) or 4.  Non-finite coordinates are outside the contract.  out [nx][4] float64. */
+int lsim_rbf_sums_grid(const float *x, const int32_t *x_event, int64_t nx, const float *y, const float *c,
+                       const int64_t *y_key, int64_t ny, const float *grid, int32_t dims, float sigma, double *out,
+                       void *stream);
+
+/* The distance beyond which lsim_rbf_sums_grid leaves pairs out (1.01 sqrt(212) sigma); callers size
+ * the grid's cells from it (any cell size is correct; about a third of it culls well). */
+float lsim_rbf_cull_radius(float sigma);
+
 /* Fit loss: event-blocked nearest neighbour of the chamfer distance (losses_jax.py:195-229
  * chamfer_distance_3d_og, :231-249 chamfer_3d): for each x_i, the index j in [lo[i], hi[i]) of the
  * y point at the smallest squared distance (dx^2 + dy^2) + dz^2 (float32; the first j on ties, as
@@ -289,10 +306,22 @@ int lsim_prob_fee_query(int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_
  * starts at tick t and its expected charge (e-).  interval = round((3 + ADC_HOLD_DELAY)
  * CLOCK_CYCLE / t_sampling) ticks.  As in the reference, all rows stop together once no row's
  * paths keep a total probability above stop_threshold; stopped steps hold -1000 and 0.
- * n_ticks <= 2048, n_paths <= 32.  Forward only. */
+ * n_ticks <= 2048, n_paths <= 32.  The workspace keeps the per-row tables and the paths of every
+ * step for lsim_prob_fee_backward. */
 int lsim_prob_fee(const float *wfs, int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, int32_t interval,
                   float threshold, float noise_sigma, float t_sampling, float stop_threshold, void *ws,
                   size_t ws_bytes, float *log_prob, float *charge, void *stream);
+
+/* The reverse-mode derivative of lsim_prob_fee (what jax.grad takes through
+ * get_adc_values_average_noise_vmap, fee_jax.py:334-461, for the llhd and probabilistic loss
+ * strategies, optimize/strategies.py:107-134, 182-567): from cotangents d_log_prob and d_charge
+ * [n_rows][n_hits][n_ticks - 1] of its two outputs, d_wfs [n_rows][n_ticks] float32.  The path
+ * selections (top_k, under stop_gradient in the reference) and the stop are held fixed.  ws and
+ * log_prob are those of the lsim_prob_fee call being differentiated, same sizes and settings. */
+int lsim_prob_fee_backward(int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, int32_t interval,
+                           float threshold, float noise_sigma, float t_sampling, const void *ws, size_t ws_bytes,
+                           const float *log_prob, const float *d_log_prob, const float *d_charge, float *d_wfs,
+                           void *stream);
 
 /* ---- Soft-DTW fit losses ------------------------------------------------------------------------- */
 

@@ -855,8 +855,35 @@ int lsim_rbf_sums(const float* x, int64_t nx, const int32_t* lo, const int32_t* 
   if (!x || !lo || !hi || !out || (ny > 0 && (!y || !c))) return fail(LSIM_E_ARG, "NULL argument");
   if (nx >= ((int64_t)1 << 40)) return fail(LSIM_E_ARG, "too many points");
   const float inv = 1.f / (2.f * sigma * sigma);
-  hipLaunchKernelGGL(k_rbf_sums, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(k_rbf_sums<4>, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                      (const float4*)x, nx, lo, hi, (const float4*)y, (const float4*)c, inv, out);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+float lsim_rbf_cull_radius(float sigma) {
+  // exp(-d^2 / (2 sigma^2)) is exactly 0 in float32 once d^2 / (2 sigma^2) > 106; 1 % margin
+  return 1.01f * sqrtf(106.f * 2.f * sigma * sigma);
+}
+
+int lsim_rbf_sums_grid(const float* x, const int32_t* x_event, int64_t nx, const float* y, const float* c,
+                       const int64_t* y_key, int64_t ny, const float* grid, int32_t dims, float sigma, double* out,
+                       void* stream) {
+  if (nx < 0 || ny < 0) return fail(LSIM_E_ARG, "negative size");
+  if (ny >= ((int64_t)1 << 31) || nx >= ((int64_t)1 << 40)) return fail(LSIM_E_ARG, "too many points");
+  if (dims != 3 && dims != 4) return fail(LSIM_E_ARG, "dims must be 3 or 4");
+  if (!(sigma > 0.f) || !std::isfinite(sigma)) return fail(LSIM_E_ARG, "sigma must be finite and > 0");
+  if (nx == 0) return LSIM_OK;
+  if (!x || !x_event || !grid || !out || (ny > 0 && (!y || !c || !y_key))) return fail(LSIM_E_ARG, "NULL argument");
+  const float inv = 1.f / (2.f * sigma * sigma);
+  const float rc = lsim_rbf_cull_radius(sigma);
+  const dim3 g((unsigned)((nx + 255) / 256));
+  if (dims == 3)
+    hipLaunchKernelGGL(k_rbf_grid<3>, g, dim3(256), 0, (hipStream_t)stream, (const float4*)x, x_event, nx,
+                       (const float4*)y, (const float4*)c, y_key, (int32_t)ny, grid, inv, rc, out);
+  else
+    hipLaunchKernelGGL(k_rbf_grid<4>, g, dim3(256), 0, (hipStream_t)stream, (const float4*)x, x_event, nx,
+                       (const float4*)y, (const float4*)c, y_key, (int32_t)ny, grid, inv, rc, out);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }
@@ -974,8 +1001,42 @@ int lsim_chop_fill(const lsim_chop_layout* layout, const float* tracks, int64_t 
 
 static size_t pf_flags_bytes(int S) { return (((size_t)(S + 1) * sizeof(int) + 255) / 256) * 256; }
 
+// flags [S + 1] | per-row tables [P][4 T] | path tape: charges, log probabilities [S + 1][P][V] and
+// selected ticks [S][P][V]
 static size_t pf_ws_bytes(int64_t P, int T, int V, int S) {
-  return pf_flags_bytes(S) + (size_t)P * 4 * T * sizeof(float) + (size_t)2 * P * V * sizeof(float);
+  return pf_flags_bytes(S) + (size_t)P * 4 * T * sizeof(float) + (size_t)2 * (S + 1) * P * V * sizeof(float) +
+         (size_t)S * P * V * sizeof(int);
+}
+
+struct PfWs {
+  int* flags;
+  float* tab;
+  float* tc;
+  float* tp;
+  int* tk;
+};
+
+static PfWs pf_layout(void* ws, int64_t P, int T, int V, int S) {
+  PfWs w;
+  w.flags = (int*)ws;
+  w.tab = (float*)((char*)ws + pf_flags_bytes(S));
+  w.tc = w.tab + (size_t)P * 4 * T;
+  w.tp = w.tc + (size_t)(S + 1) * P * V;
+  w.tk = (int*)(w.tp + (size_t)(S + 1) * P * V);
+  return w;
+}
+
+static int pf_check(int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, int32_t interval, float noise_sigma,
+                    float t_sampling) {
+  if (n_rows < 0) return fail(LSIM_E_ARG, "n_rows must be >= 0");
+  if (n_ticks < 2 || n_ticks > kPfMaxTicks) return fail(LSIM_E_ARG, "n_ticks must be in 2 .. 2048");
+  if (n_paths < 1 || n_paths > kPfMaxPaths || n_paths > n_ticks - 1)
+    return fail(LSIM_E_ARG, "n_paths must be in 1 .. min(32, n_ticks - 1)");
+  if (n_hits < 1 || n_hits > 64) return fail(LSIM_E_ARG, "n_hits must be in 1 .. 64");
+  if (interval < 0) return fail(LSIM_E_ARG, "interval must be >= 0");
+  if (!(noise_sigma > 0.f) || !(t_sampling > 0.f)) return fail(LSIM_E_ARG, "noise_sigma and t_sampling must be > 0");
+  if (n_rows >= ((int64_t)1 << 31)) return fail(LSIM_E_ARG, "too many rows");
+  return LSIM_OK;
 }
 
 int lsim_prob_fee_query(int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, size_t* ws_bytes) {
@@ -987,15 +1048,9 @@ int lsim_prob_fee_query(int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_
 int lsim_prob_fee(const float* wfs, int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, int32_t interval,
                   float threshold, float noise_sigma, float t_sampling, float stop_threshold, void* ws,
                   size_t ws_bytes, float* log_prob, float* charge, void* stream) {
-  if (n_rows < 0) return fail(LSIM_E_ARG, "n_rows must be >= 0");
-  if (n_ticks < 2 || n_ticks > kPfMaxTicks) return fail(LSIM_E_ARG, "n_ticks must be in 2 .. 2048");
-  if (n_paths < 1 || n_paths > kPfMaxPaths || n_paths > n_ticks - 1)
-    return fail(LSIM_E_ARG, "n_paths must be in 1 .. min(32, n_ticks - 1)");
-  if (n_hits < 1 || n_hits > 64) return fail(LSIM_E_ARG, "n_hits must be in 1 .. 64");
-  if (interval < 0) return fail(LSIM_E_ARG, "interval must be >= 0");
-  if (!(noise_sigma > 0.f) || !(t_sampling > 0.f) || !(stop_threshold > 0.f))
-    return fail(LSIM_E_ARG, "noise_sigma, t_sampling and stop_threshold must be > 0");
-  if (n_rows >= ((int64_t)1 << 31)) return fail(LSIM_E_ARG, "too many rows");
+  int rc = pf_check(n_rows, n_ticks, n_paths, n_hits, interval, noise_sigma, t_sampling);
+  if (rc) return rc;
+  if (!(stop_threshold > 0.f)) return fail(LSIM_E_ARG, "stop_threshold must be > 0");
   if (ws_bytes < pf_ws_bytes(n_rows, n_ticks, n_paths, n_hits)) return fail(LSIM_E_SPACE, "workspace too small");
   if (n_rows == 0) return LSIM_OK;
   if (!wfs || !ws || !log_prob || !charge) return fail(LSIM_E_ARG, "NULL argument");
@@ -1009,19 +1064,43 @@ int lsim_prob_fee(const float* wfs, int64_t n_rows, int32_t n_ticks, int32_t n_p
   d.ts = t_sampling;
   d.log_stop = logf(stop_threshold);
   hipStream_t st = (hipStream_t)stream;
-  int* flags = (int*)ws;
-  float* tab = (float*)((char*)ws + pf_flags_bytes(n_hits));
-  float* state = tab + (size_t)n_rows * 4 * n_ticks;
-  hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, st, flags, (int64_t)(n_hits + 1), 0);
-  hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, st, flags, (int64_t)1, 1);
+  const PfWs w = pf_layout(ws, n_rows, n_ticks, n_paths, n_hits);
+  hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, st, w.flags, (int64_t)(n_hits + 1), 0);
+  hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, st, w.flags, (int64_t)1, 1);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_pf_prep, dim3((unsigned)n_rows), dim3(kPfThreads), 0, st, d, wfs, n_rows, tab);
+  hipLaunchKernelGGL(k_pf_prep, dim3((unsigned)n_rows), dim3(kPfThreads), 0, st, d, wfs, n_rows, w.tab);
   HIP_TRY(hipGetLastError());
   for (int s = 0; s < n_hits; ++s) {
-    hipLaunchKernelGGL(k_pf_step, dim3((unsigned)n_rows), dim3(kPfThreads), 0, st, d, s, (const float*)tab, n_rows,
-                       state, flags, log_prob, charge);
+    hipLaunchKernelGGL(k_pf_step, dim3((unsigned)n_rows), dim3(kPfThreads), 0, st, d, s, (const float*)w.tab, n_rows,
+                       w.tc, w.tp, w.tk, w.flags, log_prob, charge);
     HIP_TRY(hipGetLastError());
   }
+  return LSIM_OK;
+}
+
+int lsim_prob_fee_backward(int64_t n_rows, int32_t n_ticks, int32_t n_paths, int32_t n_hits, int32_t interval,
+                           float threshold, float noise_sigma, float t_sampling, const void* ws, size_t ws_bytes,
+                           const float* log_prob, const float* d_log_prob, const float* d_charge, float* d_wfs,
+                           void* stream) {
+  int rc = pf_check(n_rows, n_ticks, n_paths, n_hits, interval, noise_sigma, t_sampling);
+  if (rc) return rc;
+  if (ws_bytes < pf_ws_bytes(n_rows, n_ticks, n_paths, n_hits)) return fail(LSIM_E_SPACE, "workspace too small");
+  if (n_rows == 0) return LSIM_OK;
+  if (!ws || !log_prob || !d_log_prob || !d_charge || !d_wfs) return fail(LSIM_E_ARG, "NULL argument");
+  PfDev d;
+  d.T = n_ticks;
+  d.V = n_paths;
+  d.S = n_hits;
+  d.interval = interval;
+  d.theta = threshold;
+  d.zs = 1.f / noise_sigma;
+  d.ts = t_sampling;
+  d.log_stop = 0.f;
+  const PfWs w = pf_layout(const_cast<void*>(ws), n_rows, n_ticks, n_paths, n_hits);
+  hipLaunchKernelGGL(k_pf_adj, dim3((unsigned)n_rows), dim3(kPfThreads), 0, (hipStream_t)stream, d,
+                     (const float*)w.tab, n_rows, (const float*)w.tc, (const float*)w.tp, (const int*)w.tk,
+                     (const int*)w.flags, log_prob, d_log_prob, d_charge, d_wfs);
+  HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }
 

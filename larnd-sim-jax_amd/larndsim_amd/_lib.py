@@ -18,9 +18,11 @@ MAX_ADC = 16
 EXPORTS = (
     "lsim_last_error", "lsim_version", "lsim_tape_bytes", "lsim_ctx_create", "lsim_ctx_destroy", "lsim_lut_set",
     "lsim_forward_query", "lsim_forward_prepare", "lsim_forward_rows", "lsim_forward", "lsim_fee_query",
-    "lsim_fee_forward", "lsim_fee_forward_noalloc", "lsim_negative_rows_query", "lsim_negative_rows", "lsim_backward_query", "lsim_backward", "lsim_backward_wfs", "lsim_set_row_waves", "lsim_set_row_cost", "lsim_segment_records", "lsim_timing",
-    "lsim_timing_read", "lsim_rbf_sums", "lsim_nearest", "lsim_chop_query", "lsim_chop_count", "lsim_chop_fill",
-    "lsim_prob_fee_query", "lsim_prob_fee", "lsim_sdtw_query", "lsim_sdtw",
+    "lsim_fee_forward", "lsim_fee_forward_noalloc", "lsim_negative_rows_query", "lsim_negative_rows",
+    "lsim_backward_query", "lsim_backward", "lsim_backward_wfs", "lsim_set_row_waves", "lsim_set_row_cost",
+    "lsim_segment_records", "lsim_timing", "lsim_timing_read", "lsim_rbf_sums", "lsim_rbf_sums_grid",
+    "lsim_rbf_cull_radius", "lsim_nearest", "lsim_chop_query", "lsim_chop_count", "lsim_chop_fill",
+    "lsim_prob_fee_query", "lsim_prob_fee", "lsim_prob_fee_backward", "lsim_sdtw_query", "lsim_sdtw",
 )
 
 
@@ -113,9 +115,12 @@ def load(path: str = LIB_PATH):
         "lsim_timing": [C.c_int],
         "lsim_timing_read": [C.POINTER(C.c_double), C.POINTER(i64)],
         "lsim_rbf_sums": [vp, i64, vp, vp, vp, vp, i64, C.c_float, vp, vp],
+        "lsim_rbf_sums_grid": [vp, vp, i64, vp, vp, vp, i64, vp, i32, C.c_float, vp, vp],
         "lsim_nearest": [vp, i64, vp, vp, vp, i64, i64, vp, vp, vp],
         "lsim_prob_fee_query": [i64, i32, i32, i32, C.POINTER(sz)],
         "lsim_prob_fee": [vp, i64, i32, i32, i32, i32, C.c_float, C.c_float, C.c_float, C.c_float, vp, sz, vp, vp, vp],
+        "lsim_prob_fee_backward": [i64, i32, i32, i32, i32, C.c_float, C.c_float, C.c_float, vp, sz, vp, vp, vp, vp,
+                                   vp],
         "lsim_sdtw_query": [i64, i64, C.POINTER(sz)],
         "lsim_sdtw": [vp, i64, vp, i64, C.c_float, vp, sz, vp, vp, vp],
         "lsim_chop_query": [i64, C.POINTER(sz)],
@@ -127,6 +132,8 @@ def load(path: str = LIB_PATH):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    lib.lsim_rbf_cull_radius.argtypes = [C.c_float]
+    lib.lsim_rbf_cull_radius.restype = C.c_float
     _lib = lib
     return lib
 

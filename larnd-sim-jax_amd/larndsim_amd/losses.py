@@ -99,16 +99,47 @@ def _rbf_sums_hip(xs, lo, hi, ys, c, sigma):
     return out
 
 
-def _block_sums(x, ex, y, ey, c, sigma):
+def _cell_keys(p4, e, lo, h, G):
+    """Grid key ((event * Gx + cx) * Gy + cy) * Gz + cz of each point, cells clamped to the grid."""
+    cell = torch.floor((p4[:, :3] - lo) / h)
+    cell = torch.minimum(torch.clamp(cell, min=0.0), G - 1).to(torch.int64)
+    g = G.to(torch.int64)
+    return ((e.to(torch.int64) * g[0] + cell[:, 0]) * g[1] + cell[:, 1]) * g[2] + cell[:, 2]
+
+
+def _block_sums(x, ex, y, ey, c, sigma, dims=3):
     """Per point of x: kernel-weighted sums of the 4 columns of c over the points of y in the same
-    event, on the GPU with the HIP kernel (no host synchronisation)."""
-    ox = torch.argsort(ex, stable=True)
-    oy = torch.argsort(ey, stable=True)
-    exs, eys = ex[ox].contiguous(), ey[oy].contiguous()
-    lo = torch.searchsorted(eys, exs, right=False)
-    hi = torch.searchsorted(eys, exs, right=True)
-    out = torch.empty((x.shape[0], 4), dtype=torch.float64, device=x.device)
-    out[ox] = _rbf_sums_hip(x[ox], lo, hi, y[oy], c[oy], sigma)
+    event, on the GPU (lsim_rbf_sums_grid, no host synchronisation).  Both clouds go on one grid of
+    cells about a third of the kernel's cull radius wide (at most 64 per axis), y sorted by
+    (event, cell) for the kernel's column spans and x by the same key so each wave's points are
+    close together; dims 3 ignores the fourth coordinate."""
+    import ctypes as C
+
+    from . import _lib
+
+    out = torch.zeros((x.shape[0], 4), dtype=torch.float64, device=x.device)
+    if x.shape[0] == 0 or y.shape[0] == 0:
+        return out
+    lib = _lib.load()
+    x4, y4 = _pad4(x), _pad4(y)
+    pts = torch.cat([x4[:, :3], y4[:, :3]])
+    lo = pts.amin(dim=0)
+    ext = pts.amax(dim=0) - lo
+    rc = float(lib.lsim_rbf_cull_radius(C.c_float(float(sigma))))
+    G = torch.clamp(torch.ceil(ext / (rc / 3.0)), 1.0, 64.0)
+    h = torch.where(ext > 0, ext / G, torch.ones_like(ext))
+    grid = torch.cat([lo, h, G]).to(torch.float32).contiguous()
+    kx, ky = _cell_keys(x4, ex, lo, h, G), _cell_keys(y4, ey, lo, h, G)
+    ox, oy = torch.argsort(kx, stable=True), torch.argsort(ky, stable=True)
+    xs, es = x4[ox].contiguous(), ex[ox].to(torch.int32).contiguous()
+    ys, cs, ks = y4[oy].contiguous(), c[oy].to(torch.float32).contiguous(), ky[oy].contiguous()
+    res = torch.empty((x.shape[0], 4), dtype=torch.float64, device=x.device)
+    st = C.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+    _lib.check(lib.lsim_rbf_sums_grid(C.c_void_p(xs.data_ptr()), C.c_void_p(es.data_ptr()), xs.shape[0],
+                                      C.c_void_p(ys.data_ptr()), C.c_void_p(cs.data_ptr()), C.c_void_p(ks.data_ptr()),
+                                      ys.shape[0], C.c_void_p(grid.data_ptr()), int(dims), C.c_float(float(sigma)),
+                                      C.c_void_p(res.data_ptr()), st))
+    out[ox] = res
     return out
 
 
@@ -372,7 +403,7 @@ class _NLLFn(torch.autograd.Function):
             raise RuntimeError("nll_loss runs on the GPU (lsim_rbf_sums)")
         coef = torch.zeros((c4.shape[0], 4), dtype=torch.float64, device=c4.device)
         coef[:, 0] = p.to(torch.float64)
-        lam = _block_sums(r4, er, c4, ec, coef, sigma)[:, 0]
+        lam = _block_sums(r4, er, c4, ec, coef, sigma, dims=4)[:, 0]
         m = (rp > 0.5).to(torch.float64)
         term1 = (torch.log(lam + 1e-10) * m).sum()
         ctx.save_for_backward(c4, p, r4, ec, er, lam, m)
@@ -386,10 +417,10 @@ class _NLLFn(torch.autograd.Function):
         gj = m / (lam + 1e-10)
         r64 = r4.to(torch.float64)
         coef = torch.stack((gj, gj * r64[:, 0], gj * r64[:, 1], gj * r64[:, 2]), dim=1)
-        S = _block_sums(c4, ec, r4, er, coef, ctx.sigma)
+        S = _block_sums(c4, ec, r4, er, coef, ctx.sigma, dims=4)
         coef2 = torch.zeros_like(coef)
         coef2[:, 0] = gj * r64[:, 3]
-        S3 = _block_sums(c4, ec, r4, er, coef2, ctx.sigma)[:, 0]
+        S3 = _block_sums(c4, ec, r4, er, coef2, ctx.sigma, dims=4)[:, 0]
         c64, p64 = c4.to(torch.float64), p.to(torch.float64)
         # d(-term1)/dp_i = -(K g)_i ; d(-term1)/dc_i = (p_i / s^2) (c_i (K g)_i - (K g r)_i)
         g_p = -S[:, 0]

@@ -69,8 +69,9 @@ def pad_to_closest_multiple(x, dims_to_pad=None, multiple=128, pad_value=0, pad_
 
 class HipLUTProbabilisticSimulation(SimulationStrategy):
     """LUT waveforms and the probabilistic readout on the GPU (strategies.py:107-134): the raw
-    (pixel, hit index, tick) distributions for the probabilistic losses.  The readout's adjoint is
-    not built, so parameters that require gradients are refused."""
+    (pixel, hit index, tick) distributions for the probabilistic losses.  With parameters that
+    require gradients, hit_prob and adcs_distrib carry autograd history through the readout's
+    adjoint (selections held fixed) and the waveform adjoint of the LUT simulation."""
 
     def __init__(self, response):
         self.response = response
@@ -80,11 +81,18 @@ class HipLUTProbabilisticSimulation(SimulationStrategy):
         from .fee import simulate_probabilistic
         from .sim import simulate_wfs
 
-        if needs_grad(params):
-            raise NotImplementedError("gradients through the probabilistic readout are not available")
         # the waveforms do not depend on the readout noise, which only the readout uses
-        wfs, unique_pixels = simulate_wfs(params.replace(RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0),
-                                          self.response, tracks, fields)
+        quiet = params.replace(RESET_NOISE_CHARGE=0.0, UNCORRELATED_NOISE_CHARGE=0.0)
+        if needs_grad(params):
+            sim = get_simulator(quiet, self.response)
+            theta = theta_of(quiet)
+            base = quiet.replace(**{n: float(v) for n, v in zip(
+                ("Ab", "kb", "eField", "lifetime", "vdrift", "long_diff", "tran_diff"), theta.detach().cpu().tolist())})
+            adcs, _, wfs = LUTSimFunction.apply(theta, sim, base, tracks, fields, infer_n_events(tracks, fields))
+            o = _last_others(adcs)
+            wfs, unique_pixels = reference_rows(wfs, o["unique_pixels"], *o["negative_rows"])
+        else:
+            wfs, unique_pixels = simulate_wfs(quiet, self.response, tracks, fields)
         unique_pixels = pad_to_closest_multiple(unique_pixels, multiple=128, pad_value=-1, pad_front=True)
         wfs = pad_to_closest_multiple(wfs, dims_to_pad=(0,), multiple=128, pad_value=0.0, pad_front=True)
         adcs_distrib, pixel_x, pixel_y, ticks_prob, event = simulate_probabilistic(params, wfs, unique_pixels)

@@ -115,6 +115,3 @@ def test_probabilistic_strategy(params4, lut, c1):
     lp, q = get_adc_values_average_noise_vmap(p, out["wfs"])
     assert torch.equal(lp, out["hit_prob"])
     assert torch.equal(digitize(p, q), out["adcs_distrib"])
-    with pytest.raises(NotImplementedError):
-        HipLUTProbabilisticSimulation(lut).predict(
-            p.replace(Ab=torch.tensor(float(p.Ab), dtype=torch.float64, requires_grad=True)), tracks, fields)

@@ -253,6 +253,24 @@ def test_new_entry_points_check_arguments(lib):
     assert lib.lsim_prob_fee(None, 1, 2000, 20, 10, 18, C.c_float(7e3), C.c_float(0.0), C.c_float(0.1),
                              C.c_float(1e-9), None, 1 << 30, None, None, None) < 0
     assert b"noise_sigma" in lib.lsim_last_error()
+    # the readout's adjoint: same limits, a workspace of the forward's size, no rows is a no-op
+    bwd = lambda P, T, V, ws_bytes: lib.lsim_prob_fee_backward(  # noqa: E731
+        P, T, V, 10, 18, C.c_float(7e3), C.c_float(900.0), C.c_float(0.1), None, ws_bytes, None, None, None, None, None)
+    assert bwd(1, 4096, 20, 1 << 30) < 0 and bwd(1, 2000, 33, 1 << 30) < 0
+    assert lib.lsim_prob_fee_query(10, 2000, 20, 10, C.byref(sz)) == 0
+    assert bwd(10, 2000, 20, sz.value - 1) == -4 and b"workspace" in lib.lsim_last_error()
+    assert bwd(10, 2000, 20, sz.value) < 0 and b"NULL" in lib.lsim_last_error()
+    assert bwd(0, 2000, 20, 1 << 20) == 0
+    # grid kernel sums: the cull radius is where exp(-d^2 / 2 sigma^2) is 0 in float32 (e^-106), +1 %
+    assert lib.lsim_rbf_cull_radius(C.c_float(2.0)) == pytest.approx(1.01 * np.sqrt(212.0) * 2.0, rel=1e-6)
+    assert float(np.float32(np.exp(-106.0))) == 0.0
+    grid = lambda nx, ny, dims, sigma: lib.lsim_rbf_sums_grid(  # noqa: E731
+        C.c_void_p(8), C.c_void_p(8), nx, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8), ny, C.c_void_p(8), dims,
+        C.c_float(sigma), C.c_void_p(8), None)
+    assert grid(10, 10, 5, 1.0) < 0 and b"dims" in lib.lsim_last_error()
+    assert grid(10, 10, 3, 0.0) < 0 and grid(10, 10, 3, float("inf")) < 0
+    assert grid(10, 1 << 31, 3, 1.0) < 0 and grid(-1, 10, 3, 1.0) < 0
+    assert grid(0, 10, 3, 1.0) == 0
     assert lib.lsim_sdtw_query(100, 200, C.byref(sz)) == 0 and sz.value >= 2 * 100 * 200 * 4
     assert lib.lsim_sdtw_query(0, 5, C.byref(sz)) < 0
     assert lib.lsim_sdtw_query(1 << 20, 1 << 15, C.byref(sz)) < 0  # n * m beyond 2^34
