@@ -858,18 +858,51 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
 // the row's (2e+1)^2 neighbourhood cells plus a fixed per-row cost (row set-up, sum, cumulative
 // charge and self-trigger) in the same units.
 
-// Estimated work of every row: one thread per row.  Rows P..max_rows-1 hold no pixel: their hit
-// count is zeroed here for the hit scan over max_rows (the row kernel never visits them).
-__global__ void k_row_work(Dev d, const int2* __restrict__ rng, const int* __restrict__ row_cell,
-                           const int* __restrict__ n_rows_dev, int64_t max_rows, int row_cost, int* __restrict__ work,
-                           int* __restrict__ hit_cnt) {
+// Processing order of the pixel rows: by tile of kRowTile x kRowTile cells of one event plane, in
+// row order within a tile (a stable sort on the tile id).  Consecutive workgroups of an XCD then
+// work on neighbouring pixels, whose neighbourhoods share most of their segments, so the segment
+// records and corrections a band reads stay in that XCD's L2.  The order changes which workgroup
+// computes a row and when, never what it computes.
+#ifndef LSIM_ROW_TILE
+#define LSIM_ROW_TILE 16
+#endif
+constexpr int kRowTile = LSIM_ROW_TILE;
+
+inline int64_t row_tiles(const Dev& d) {
+  const int64_t ntx = (d.gx + kRowTile - 1) / kRowTile, nty = (d.gy + kRowTile - 1) / kRowTile;
+  return d.n_cells / ((int64_t)d.gx * d.gy) * ntx * nty;
+}
+
+// Sort keys of the rows: the tile id, or `past` for the rows P..max_rows-1 (after every tile).
+__global__ void k_tile_keys(Dev d, const int* __restrict__ row_cell, const int* __restrict__ n_rows_dev,
+                            int64_t max_rows, uint32_t past, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= max_rows) return;
   const int64_t P = *n_rows_dev < max_rows ? *n_rows_dev : max_rows;
+  vals[r] = (uint32_t)r;
   if (r >= P) {
-    if (r < max_rows) hit_cnt[r] = 0;
+    keys[r] = past;
     return;
   }
   const int cell = row_cell[r];
+  const int cx = cell % d.gx, cy = (cell / d.gx) % d.gy, evp = cell / (d.gx * d.gy);
+  const uint32_t ntx = (uint32_t)((d.gx + kRowTile - 1) / kRowTile), nty = (uint32_t)((d.gy + kRowTile - 1) / kRowTile);
+  keys[r] = ((uint32_t)evp * nty + (uint32_t)(cy / kRowTile)) * ntx + (uint32_t)(cx / kRowTile);
+}
+
+// Estimated work of every row, in processing order (work[s] for the row order[s]): one thread per
+// slot.  Rows P..max_rows-1 hold no pixel (and keep their place in the order): their hit count is
+// zeroed here for the hit scan over max_rows (the row kernel never visits them).
+__global__ void k_row_work(Dev d, const int2* __restrict__ rng, const int* __restrict__ row_cell,
+                           const int* __restrict__ order, const int* __restrict__ n_rows_dev, int64_t max_rows,
+                           int row_cost, int* __restrict__ work, int* __restrict__ hit_cnt) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t P = *n_rows_dev < max_rows ? *n_rows_dev : max_rows;
+  if (s >= P) {
+    if (s < max_rows) hit_cnt[s] = 0;
+    return;
+  }
+  const int cell = row_cell[order[s]];
   const int px = cell % d.gx - d.halo;
   const int py = (cell / d.gx) % d.gy - d.halo;
   const int evp = cell / (d.gx * d.gy);
@@ -884,7 +917,7 @@ __global__ void k_row_work(Dev d, const int2* __restrict__ rng, const int* __res
       n += rg.y - rg.x;
     }
   }
-  work[r] = n;
+  work[s] = n;
 }
 
 // Band starts band[0..kXcd] (band[kXcd] = P) at equal shares of the total work; one workgroup of
@@ -949,14 +982,15 @@ constexpr int kFarGroup = LSIM_FAR_GROUP;  // independent neighbour-only pairs i
 
 // XCD-aware, work-balanced mapping of workgroups onto rows: workgroups are dispatched round-robin
 // over the XCDs, so workgroup i runs on XCD x = i % kXcd; it takes slot i / kXcd of band x (k_bands:
-// contiguous rows cut at equal shares of the estimated work, so a band's records and tables stay
-// in that XCD's L2 and every XCD finishes at about the same time).  The grid holds kXcd times the
+// contiguous slots of the tile-major processing order, cut at equal shares of the estimated work,
+// so a band's records and tables stay in that XCD's L2 and every XCD finishes at about the same
+// time), and computes row order[slot].  The grid holds kXcd times the
 // longest band allowed (rows_grid); the workgroups past their band's end exit at once.
 template <int NB, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const int2* __restrict__ rng, const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
-                                                            const int* __restrict__ band,
+                                                            const int* __restrict__ band, const int* __restrict__ order,
                                                             const float* __restrict__ vc, const float* __restrict__ dtot,
                                                             const PairRec* __restrict__ prs, int64_t nseg, RowOut out) {
   constexpr int NT = WAVES * kWave;
@@ -970,8 +1004,9 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   const int P0 = *n_rows_dev;
   const int P = P0 < max_rows ? P0 : (int)max_rows;
   const int xcd = (int)(blockIdx.x % kXcd);
-  const int r = band[xcd] + (int)(blockIdx.x / kXcd);
-  if (r >= band[xcd + 1] || r >= P) return;  // past the band (the bound on P is a guard only)
+  const int slot = band[xcd] + (int)(blockIdx.x / kXcd);
+  if (slot >= band[xcd + 1] || slot >= P) return;  // past the band (the bound on P is a guard only)
+  const int r = order[slot];
   const int nout = d.nout;
   float* acc = (float*)smem;
   const int t = threadIdx.x;
@@ -1053,26 +1088,39 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
       const int n = __builtin_amdgcn_readfirstlane(nfar - b0 < kWave ? nfar - b0 : kWave);
       float* base = accw - 2;
       const float m0 = lane == 0 ? 1.f : 0.f, m1 = lane == 1 ? 1.f : 0.f;
-      for (int i = 0; i < n; i += kFarGroup) {
+      // pairs i .. i + kFarGroup - 1 of the block: their factors from the owning lanes, their window
+      // loads issued (j < 64: i < n <= 64 and i is a multiple of kFarGroup)
+      struct FarGroup {
         FarLoads q[kFarGroup];
         int S[kFarGroup];
         float so[kFarGroup], sf[kFarGroup], fD[kFarGroup], omfD[kFarGroup];
+      };
+      auto issue = [&](int i, FarGroup& G) {
 #pragma unroll
         for (int g = 0; g < kFarGroup; ++g) {
-          const int j = i + g;  // < 64: i < n <= 64 and i is a multiple of kFarGroup
-          S[g] = __builtin_amdgcn_readlane(rS, j);
-          so[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rso), j));
-          sf[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rsf), j));
-          fD[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rfD), j));
-          omfD[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(romfD), j));
-          far_load_buf(tnp, lane * 4, __builtin_amdgcn_readlane(roff, j), q[g]);
+          const int j = i + g;
+          G.S[g] = __builtin_amdgcn_readlane(rS, j);
+          G.so[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rso), j));
+          G.sf[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rsf), j));
+          G.fD[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rfD), j));
+          G.omfD[g] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(romfD), j));
+          far_load_buf(tnp, lane * 4, __builtin_amdgcn_readlane(roff, j), G.q[g]);
         }
-        // unconditional stores: a branch on the charge would let the compiler sink a pair's loads
-        // into it and wait on them there instead of overlapping them with the other pairs'
+      };
+      // unconditional stores: a branch on the charge would let the compiler sink a pair's loads
+      // into it and wait on them there instead of overlapping them with the other pairs'
+      auto drain = [&](const FarGroup& G) {
 #pragma unroll
         for (int g = 0; g < kFarGroup; ++g)
-          store_stripe(base, lane, m0, m1, q[g].t0, q[g].t1, q[g].t2, so[g], sf[g], fD[g], omfD[g], S[g] >> 1, d.L,
-                       d.nout, (S[g] & 1) != 0);
+          store_stripe(base, lane, m0, m1, G.q[g].t0, G.q[g].t1, G.q[g].t2, G.so[g], G.sf[g], G.fD[g], G.omfD[g],
+                       G.S[g] >> 1, d.L, d.nout, (G.S[g] & 1) != 0);
+      };
+      // (issuing the next group's loads before this group's stores, or 8-pair groups, measured no
+      // faster on the MI355X: the loads are not what the waves wait on)
+      for (int i = 0; i < n; i += kFarGroup) {
+        FarGroup A;
+        issue(i, A);
+        drain(A);
       }
       for (uint64_t m = __ballot(bins); m != 0; m &= m - 1) {
         const int j = __builtin_ctzll(m);

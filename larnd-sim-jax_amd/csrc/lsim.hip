@@ -181,8 +181,9 @@ struct Ws {
   int* hit_off;
   int* scan_tmp;
   int* n_hits32;
-  int* row_work;  // [n] estimated work of each pixel row (XCD band cuts of the row kernel)
-  int* band;      // [kXcd + 1] first row of each XCD's band
+  int* row_work;  // [n] estimated work of each pixel row in processing order (XCD band cuts)
+  int* band;      // [kXcd + 1] first processing slot of each XCD's band
+  uint32_t *tk0, *tv0, *tk1, *tv1;  // [n + 1] tile sort ping-pong; the processing order lands in tv0 or tv1
   float* vc;    // [n][kWinPad] centre-pixel windows of each segment (sorted order, padded)
   float* dtot;  // [(2 nn + 1)^2][n] boundary corrections of every (segment, row offset) pair
   size_t bytes;
@@ -195,7 +196,7 @@ Ws layout(void* base, int64_t n, int64_t n_cells) {
   w.v0 = c.take<uint32_t>(n);
   w.k1 = c.take<uint32_t>(n);
   w.v1 = c.take<uint32_t>(n);
-  w.sort_scratch = c.take<char>((int64_t)radix_sort_scratch(n));
+  w.sort_scratch = c.take<char>((int64_t)radix_sort_scratch(n + 1));  // segments, then rows (<= max(n, 1))
   w.rec = c.take<SegRec>(n);
   w.prs = c.take<PairRec>(n);
   w.rng = c.take<int2>(n_cells);
@@ -210,6 +211,10 @@ Ws layout(void* base, int64_t n, int64_t n_cells) {
   w.n_hits32 = c.take<int>(4);
   w.row_work = c.take<int>(n);
   w.band = c.take<int>(kXcd + 1);
+  w.tk0 = c.take<uint32_t>(n + 1);
+  w.tv0 = c.take<uint32_t>(n + 1);
+  w.tk1 = c.take<uint32_t>(n + 1);
+  w.tv1 = c.take<uint32_t>(n + 1);
   w.vc = nullptr;
   w.dtot = nullptr;
   w.bytes = c.off + 256;
@@ -251,13 +256,31 @@ int g_row_cost = 256;
 // is kXcd times this, so at most a third of its workgroups find no row).
 int64_t band_slots(int64_t max_rows) { return (3 * max_rows + 2 * kXcd - 1) / (2 * kXcd); }
 
-// Work estimate of every row and the XCD band cuts (k_row_work, k_bands) before the row kernel;
-// also zeroes the hit counts of the rows past P.
-void launch_row_bands(const Dev& d, const Ws& w, int64_t max_rows, hipStream_t st) {
-  hipLaunchKernelGGL(k_row_work, dim3((unsigned)((max_rows + 255) / 256)), dim3(256), 0, st, d, w.rng, w.row_cell,
-                     w.n_rows, max_rows, g_row_cost, w.row_work, w.hit_cnt);
+// Bits of the tile-order sort key: every tile id and the key of the rows past P.
+int tile_key_bits(const Dev& d) { return bits_for(row_tiles(d) + 1); }
+
+// The processing order of the rows (sorted by tile: k_tile_keys + radix sort); the buffer it lands
+// in depends only on the key width, like sorted_buffers.
+const int* row_order(const Ws& w, const Dev& d) {
+  return (const int*)((((tile_key_bits(d) + 7) / 8) & 1) ? w.tv1 : w.tv0);
+}
+
+// Processing order, work estimate of every row and the XCD band cuts (k_tile_keys + sort,
+// k_row_work, k_bands) before the row kernel; also zeroes the hit counts of the rows past P.
+int launch_row_bands(const Dev& d, const Ws& w, int64_t max_rows, hipStream_t st) {
+  const int bits = tile_key_bits(d);
+  const unsigned g = (unsigned)((max_rows + 255) / 256);
+  hipLaunchKernelGGL(k_tile_keys, dim3(g), dim3(256), 0, st, d, w.row_cell, w.n_rows, max_rows,
+                     (uint32_t)(((uint64_t)1 << bits) - 1), w.tk0, w.tv0);
+  hipError_t err;
+  radix_sort_pairs(w.tk0, w.tv0, w.tk1, w.tv1, max_rows, bits, w.sort_scratch, st, &err);
+  HIP_TRY(err);
+  hipLaunchKernelGGL(k_row_work, dim3(g), dim3(256), 0, st, d, w.rng, w.row_cell, row_order(w, d), w.n_rows, max_rows,
+                     g_row_cost, w.row_work, w.hit_cnt);
   hipLaunchKernelGGL(k_bands, dim3(1), dim3(kBandThreads), 0, st, w.row_work, w.n_rows, max_rows, band_slots(max_rows),
                      w.band);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
 }
 
 template <int NB, int WAVES>
@@ -266,7 +289,7 @@ void launch_rows_nb(const Dev& d, const Tables& tb, const Ws& w, int64_t n_seg, 
   const size_t lds = rows_lds_bytes(WAVES, d.nout);
   const int64_t grid = kXcd * band_slots(max_rows);
   hipLaunchKernelGGL((k_rows_fwd<NB, WAVES>), dim3((unsigned)grid), dim3(WAVES * kWave), lds, st, d, tb, w.rec, w.rng,
-                     w.row_cell, w.n_rows, max_rows, w.band, w.vc, w.dtot, w.prs, n_seg, ro);
+                     w.row_cell, w.n_rows, max_rows, w.band, row_order(w, d), w.vc, w.dtot, w.prs, n_seg, ro);
 }
 
 template <int WAVES>
@@ -409,23 +432,34 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
   c->nt = nt;
   const int64_t ms = (int64_t)k.n_templates * c->cm * c->cm, ns = (int64_t)nx * ny;
   const int L = k.signal_length;
-  if (!c->has_lut) {
-    HIP_TRY(hipMalloc(&c->TM3, ms * kWinPad * sizeof(float3)));
-    HIP_TRY(hipMalloc(&c->TM, ms * L * sizeof(float)));
-    HIP_TRY(hipMalloc(&c->TN, ns * L * sizeof(float)));
-    HIP_TRY(hipMalloc(&c->TNP, ns * kWinPad * sizeof(float)));
-    HIP_TRY(hipMalloc(&c->DM, ms * nt * sizeof(float2)));
-    HIP_TRY(hipMalloc(&c->DN, ns * nt * sizeof(float2)));
-    HIP_TRY(hipMalloc(&c->CTM, ms * (L + 1) * sizeof(double)));
-    HIP_TRY(hipMalloc(&c->CTN, ns * (L + 1) * sizeof(double)));
-  }
   Dev d;
   int rc = make_dev(c, nullptr, 1, d);
   if (rc) return rc;
+  if (!c->has_lut) {
+    // all tables or none: a failed allocation frees the ones made before it (free_tables resets
+    // every pointer, and hipFree of a null pointer is a no-op)
+    const std::pair<void**, size_t> req[] = {
+        {(void**)&c->TM3, ms * kWinPad * sizeof(float3)}, {(void**)&c->TM, ms * L * sizeof(float)},
+        {(void**)&c->TN, ns * L * sizeof(float)},         {(void**)&c->TNP, ns * kWinPad * sizeof(float)},
+        {(void**)&c->DM, ms * nt * sizeof(float2)},       {(void**)&c->DN, ns * nt * sizeof(float2)},
+        {(void**)&c->CTM, ms * (L + 1) * sizeof(double)}, {(void**)&c->CTN, ns * (L + 1) * sizeof(double)}};
+    for (const auto& q : req) {
+      const hipError_t e = hipMalloc(q.first, q.second);
+      if (e != hipSuccess) {
+        *q.first = nullptr;
+        free_tables(c);
+        return fail(LSIM_E_HIP, std::string("hipMalloc of the LUT tables: ") + hipGetErrorString(e));
+      }
+    }
+  }
   hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ms), dim3(256), 0, st, d, raw, 1, c->TM, (float*)nullptr, c->DM, c->CTM);
   hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ns), dim3(256), 0, st, d, raw, 0, c->TN, c->TNP, c->DN, c->CTN);
   hipLaunchKernelGGL(k_lut_tm3, dim3(1024), dim3(256), 0, st, d, c->TM, c->TM3);
-  HIP_TRY(hipGetLastError());
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    free_tables(c);  // no half-built tables stay behind (and none leak on the next call)
+    return fail(LSIM_E_HIP, std::string("LUT table kernels: ") + hipGetErrorString(e));
+  }
   c->has_lut = true;
   return LSIM_OK;
 }
@@ -520,7 +554,8 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_s
     }
   }
   if (max_rows > 0) {
-    launch_row_bands(d, w, max_rows, st);
+    const int rc = launch_row_bands(d, w, max_rows, st);
+    if (rc) return rc;
     hipEvent_t stop_ev;
     timer_begin(st, &stop_ev);
     switch (g_row_waves) {
