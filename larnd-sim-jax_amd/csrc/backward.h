@@ -283,115 +283,6 @@ __device__ __forceinline__ void g_dots(const RowG& G, const double* __restrict__
   }
 }
 
-// The same window sums for a dense G (one breakpoint per tick, G(o) = c[o]: the telescoped form is
-// then no cheaper than the direct one), split over a group of GL lanes working on one pair: lane l
-// of the group takes the window's samples j = l, l + GL, ... straight from the float windows X_k
-// (the forward's own samples, whose prefix sums CX_k are); y1 = sum_{o in [lo1, hi1]} G(o) X[o - S],
-// y0 = sum_{o in [lo0, hi0]} G(o) X[o - S + 1].  Each lane returns its own share: the callers'
-// results are linear in (y1, y0), so the shares are added by the block's closing reduction, with no
-// cross-lane step here.
-//
-// kLaneUnroll samples per lane are loaded before any is used, with no branch around the loads, so
-// they are in flight together: a sample past the window (j >= L) reads sample L - 1 and a tick
-// outside the row reads a clamped entry of G, and both get weight 0 (j >= L puts o past hi1 and
-// hi0).  Each lane still adds its samples in increasing j.
-#ifndef LSIM_LANE_UNROLL
-#define LSIM_LANE_UNROLL 4
-#endif
-constexpr int kLaneUnroll = LSIM_LANE_UNROLL;
-
-struct LaneTick {
-  int jl;         // sample index to load (clamped into the window)
-  float c1, c0;   // G at o1 = S + j and o0 = S - 1 + j (clamped ticks)
-  bool in1, in0;  // o1 in [lo1, hi1], o0 in [lo0, hi0] (false past the window)
-};
-
-__device__ __forceinline__ LaneTick lane_tick(const RowG& G, const PairCtx& c, int L, int j) {
-  LaneTick t;
-  const int o1 = c.S + j, o0 = o1 - 1, top = G.np - 1;
-  t.jl = j < L ? j : L - 1;
-  t.in1 = j < L && o1 >= c.lo1 && o1 <= c.hi1;
-  t.in0 = j < L && o0 >= c.lo0 && o0 <= c.hi0;
-  t.c1 = G.c[o1 < 0 ? 0 : (o1 > top ? top : o1)];
-  t.c0 = G.c[o0 < 0 ? 0 : (o0 > top ? top : o0)];
-  return t;
-}
-
-template <int NT, int GL>
-__device__ __forceinline__ void g_dots_lanes(const RowG& G, const float* __restrict__ X, int64_t xs,
-                                             const PairCtx& c, int L, double* y1, double* y0) {
-  const int gl = threadIdx.x & (GL - 1);
-#pragma unroll
-  for (int k = 0; k < NT; ++k) y1[k] = y0[k] = 0.0;
-  for (int j0 = gl; j0 < L; j0 += kLaneUnroll * GL) {
-    LaneTick tk[kLaneUnroll];
-    float x[kLaneUnroll][NT];
-#pragma unroll
-    for (int u = 0; u < kLaneUnroll; ++u) {
-      tk[u] = lane_tick(G, c, L, j0 + u * GL);
-#pragma unroll
-      for (int k = 0; k < NT; ++k) x[u][k] = X[k * xs + tk[u].jl];
-    }
-#pragma unroll
-    for (int u = 0; u < kLaneUnroll; ++u) {
-      const double g1 = tk[u].in1 ? (double)tk[u].c1 : 0.0;
-      const double g0 = tk[u].in0 ? (double)tk[u].c0 : 0.0;
-#pragma unroll
-      for (int k = 0; k < NT; ++k) {
-        y1[k] += g1 * (double)x[u][k];
-        y0[k] += g0 * (double)x[u][k];
-      }
-    }
-  }
-}
-
-// The three templates m - 1, m, m + 1 of a diffusion-bin cell at once, from the interleaved
-// windows X3[j] = (T_{m-1}, T_m, T_{m+1})[j] (one 12-byte load per sample).
-template <int GL>
-__device__ __forceinline__ void g_dots_lanes3(const RowG& G, const float3* __restrict__ X3, const PairCtx& c, int L,
-                                              double* y1, double* y0) {
-  const int gl = threadIdx.x & (GL - 1);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) y1[k] = y0[k] = 0.0;
-  for (int j0 = gl; j0 < L; j0 += kLaneUnroll * GL) {
-    LaneTick tk[kLaneUnroll];
-    float3 x[kLaneUnroll];
-#pragma unroll
-    for (int u = 0; u < kLaneUnroll; ++u) {
-      tk[u] = lane_tick(G, c, L, j0 + u * GL);
-      x[u] = X3[tk[u].jl];
-    }
-#pragma unroll
-    for (int u = 0; u < kLaneUnroll; ++u) {
-      const double g1 = tk[u].in1 ? (double)tk[u].c1 : 0.0;
-      const double g0 = tk[u].in0 ? (double)tk[u].c0 : 0.0;
-      y1[0] += g1 * (double)x[u].x;
-      y0[0] += g0 * (double)x[u].x;
-      y1[1] += g1 * (double)x[u].y;
-      y0[1] += g0 * (double)x[u].y;
-      y1[2] += g1 * (double)x[u].z;
-      y0[2] += g0 * (double)x[u].z;
-    }
-  }
-}
-
-// GL = 0: one thread per pair, telescoped over the prefix sums CX; else GL lanes per pair over the
-// windows X (stride xs), each returning its share.
-template <int NT, int GL>
-__device__ __forceinline__ void g_dots_any(const RowG& G, const double* __restrict__ CX, int64_t ts,
-                                           const float* __restrict__ X, int64_t xs, const PairCtx& c, int L, double* y1,
-                                           double* y0) {
-  if constexpr (GL > 0) g_dots_lanes<NT, GL>(G, X, xs, c, L, y1, y0);
-  else g_dots<NT>(G, CX, ts, c, L, y1, y0);
-}
-
-// Weight of the terms that do not come from the window sums: 1 on one lane of each pair's group.
-template <int GL>
-__device__ __forceinline__ double lead_lane() {
-  if constexpr (GL > 0) return (threadIdx.x & (GL - 1)) == 0 ? 1.0 : 0.0;
-  else return 1.0;
-}
-
 // Distinct LUT cells of the diffusion bins that land on the row along one axis (pixel offset a):
 // bins with the same cell have their weights summed (the bin weights factorise, w = wx wy, and
 // every other factor depends on the cell only), so each distinct (cx, cy) is visited once.
@@ -450,17 +341,15 @@ __device__ __forceinline__ void pair_ctx(const Dev& d, int S, float ne, float f,
 }
 
 // The neighbour stripe of pixel offset (a, b) != (0, 0) of a segment at sub-bins (ux, uy):
-// d loss / d n_e (gN) and / d f (gF).  GL > 0: this lane's share of a GL-lane group's pair.
-template <int GL = 0>
+// d loss / d n_e (gN) and / d f (gF).
 __device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, int ux, int uy, const PairCtx& c, int a,
                                                int b, const RowG& G, double& gN, double& gF) {
   const int L = d.L, nsb = d.nsb;
   const int64_t sidx = (int64_t)offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
   double y1, y0;
-  g_dots_any<1, GL>(G, tb.CTN + sidx * (L + 1), 0, tb.TN + sidx * L, 0, c, L, &y1, &y0);
+  g_dots<1>(G, tb.CTN + sidx * (L + 1), 0, c, L, &y1, &y0);
   const float2 D = tb.DN[(int64_t)(d.nt - L - c.S) * d.lx * d.ly + sidx];  // tick-major, k0 = nt - L - S
-  const double u = lead_lane<GL>();
-  const double corr = u * ((double)D.x - c.f * (double)D.y), dy = u * (double)D.y;
+  const double corr = (double)D.x - c.f * (double)D.y, dy = (double)D.y;
   gN += (1.0 - c.f) * y1 + c.f * y0 + c.gc * corr;
   gF += c.ne * (y0 - y1 + (c.gc1 - c.gc0) * corr - c.gc * dy);
 }
@@ -469,7 +358,7 @@ __device__ __forceinline__ void stripe_adjoint(const Dev& d, const Tables& tb, i
 // landing on pixel offset (a, b), |a|, |b| <= 1: d loss / d (n_e, f, sigma_T, lambda) summed over
 // the y bins (only y-bin jsel when jsel >= 0).  false (nothing added) when bin i misses column a
 // or an earlier x-bin has its cell.
-template <int NB, int GL = 0>
+template <int NB>
 __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, const SegRec& R, const AdjRec* __restrict__ Ap,
                                              const PairCtx& c, int a, int b, int i, int jsel, const RowG& G, double& gN,
                                              double& gF, double& gS, double& gL) {
@@ -498,16 +387,14 @@ __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, con
   const double la = (double)R.la, lb = (double)R.lb, lc = (double)R.lc;
   const double da = Ap->dl[0], db = Ap->dl[1], dc = Ap->dl[2];
   const double f = c.f, ne = c.ne;
-  const double u = lead_lane<GL>();
 #pragma unroll
   for (int jj = 0; jj < NB; ++jj) {
     if (!oky[jj] || (jsel >= 0 && jj != jsel)) continue;
     const int64_t cell = ((int64_t)R.idx * cm + cx) * cm + cyv[jj];
     double y1[3], y0[3];
-    if constexpr (GL > 0) g_dots_lanes3<GL>(G, tb.TM3 + cell * kWinPad + 2, c, L, y1, y0);  // window at p = 2
-    else g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c, L, y1, y0);
+    g_dots<3>(G, tb.CTM + (cell - (int64_t)cm * cm) * (L + 1), tstride, c, L, y1, y0);
     const float2 D = tb.DM[(int64_t)R.k0 * d.nm * cm * cm + cell];  // tick-major
-    const double corr = u * ((double)D.x - f * (double)D.y), dy = u * (double)D.y;
+    const double corr = (double)D.x - f * (double)D.y, dy = (double)D.y;
     const double wxy = (double)WXi * (double)WY[jj];
     const double dwxy = (double)DWXi * (double)WY[jj] + (double)WXi * (double)DWY[jj];
     const double Y1 = la * y1[0] + lb * y1[1] + lc * y1[2];
@@ -535,13 +422,6 @@ __device__ __forceinline__ void add_chain(const AdjRec* __restrict__ Ap, double 
   }
 }
 
-#ifndef LSIM_DENSE_LANES
-#define LSIM_DENSE_LANES 8
-#endif
-constexpr int kDenseLanes = LSIM_DENSE_LANES;  // lanes per pair item of the dense (waveform-cotangent) adjoint
-static_assert(kDenseLanes > 0 && kDenseLanes <= 64 && (kDenseLanes & (kDenseLanes - 1)) == 0,
-              "lanes per dense item: a power of two up to the wave");
-
 #ifndef LSIM_ADJ_BIN_ITEMS
 #define LSIM_ADJ_BIN_ITEMS 0
 #endif
@@ -552,21 +432,18 @@ constexpr bool kAdjBinItems = LSIM_ADJ_BIN_ITEMS != 0;  // one work item per dif
 // every thread taking pairs from the flattened list; fixed-order block reduction to the row's 7
 // partial gradients.  Two passes (separate launches, separate partials): NEAR = false takes the
 // neighbour-only cells (|a| or |b| > 1; little register state, high occupancy), NEAR = true the
-// 3x3 cells with the diffusion bins.  Rows without gradient (no valid hit) exit at once.
-// DENSE: the row's cotangent is a full waveform cotangent gw[row][o] (plus the self-trigger's step
-// function): one breakpoint per output tick, the same telescoped window sums with up to L
-// breakpoints per window.
-template <int NB, int WAVES, bool NEAR, bool DENSE>
+// 3x3 cells with the diffusion bins.  Rows without gradient (no valid hit) exit at once.  (A
+// waveform cotangent takes k_rows_adj_dense instead.)
+template <int NB, int WAVES, bool NEAR>
 __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_rows_adj(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const PairRec* __restrict__ prs,
                                                             const AdjRec* __restrict__ adj, const int2* __restrict__ rng,
                                                             const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
                                                             const Pieces* __restrict__ pieces,
-                                                            const float* __restrict__ gw,
                                                             double* __restrict__ partial) {
   constexpr int NT = WAVES * kWave;
-  constexpr int kNP = DENSE ? kMaxOut : 2 * kMaxH;  // breakpoints held in LDS
+  constexpr int kNP = 2 * kMaxH;  // breakpoints held in LDS
   __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
   __shared__ int ncls[3];
   __shared__ int cell_scratch[3 * kMaxCells];
@@ -586,24 +463,8 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
   const int q8 = P / kXcd, rem = P % kXcd;
   const int r = xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
   const Pieces* pr = pieces + r;
-  int np;
-  if (DENSE) {
-    // G(o) = gw[r][o] + the self-trigger's step function at o, one breakpoint per output tick
-    np = d.nout;
-    const float* gr = gw + (int64_t)r * d.nout;
-    const int nf = pr->n;
-    for (int o = t; o < np; o += NT) {
-      float cf = 0.f;
-      for (int k = nf - 1; k >= 0; --k)
-        if (pr->x[k] >= o) cf = pr->c[k];
-      spx[o] = o;
-      spc[o] = gr[o] + cf;
-    }
-    __syncthreads();
-    for (int o = t; o < np; o += NT) sdc[o] = spc[o] - (o + 1 < np ? spc[o + 1] : 0.f);
-    for (int o = t; o < kMaxOut; o += NT) qlut[o] = (uint16_t)(o < np ? o : np);
-  } else {
-    np = pr->n;
+  const int np = pr->n;
+  {
     if (np == 0) {
       if (t < kNGrad) partial[(int64_t)r * kNGrad + t] = 0.0;
       return;
@@ -652,40 +513,7 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
   constexpr int NI = NEAR ? NX + 1 : 1;
   int cur = 0;
   const int lane = t & (kWave - 1), wid = t / kWave;
-  if (DENSE) {
-    // kDenseLanes lanes per item, splitting its windows' samples; every lane adds its share
-    constexpr int GL = kDenseLanes;
-    for (int it = t / GL; it < tot * NI; it += NT / GL) {
-      const int i = it / NI, part = it - i * NI;
-      while (i >= c_cum[cur + 1]) ++cur;
-      const int s = c_beg[cur] + (i - c_cum[cur]);
-      const int ab = c_ab[cur];
-      const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
-      double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
-      PairCtx pc;
-      if (!NEAR) {
-        const int4 P = *reinterpret_cast<const int4*>(prs + s);
-        const float ne = __int_as_float(P.y);
-        if (ne == 0.f) continue;
-        pair_ctx(d, P.x, ne, __int_as_float(P.z), G, pc);
-        stripe_adjoint<GL>(d, tb, (P.w >> 8) & 255, (P.w >> 16) & 255, pc, a, b, G, gN, gF);
-      } else {
-        const SegRec R = rec[s];
-        if (R.ne == 0.f) continue;
-        const int S = d.nt - d.L - R.k0;
-        if (pair_dead(S, G)) continue;
-        pair_ctx(d, S, R.ne, R.f, G, pc);
-        if (part == NX) {
-          if ((a == 0 && b == 0) || a < -d.nn || a > d.nn || b < -d.nn || b > d.nn) continue;
-          stripe_adjoint<GL>(d, tb, R.uxy & 0xffff, R.uxy >> 16, pc, a, b, G, gN, gF);
-        } else if (!xbin_adjoint<NB, GL>(d, tb, R, adj + s, pc, a, b, kAdjBinItems ? part / NB : part,
-                                           kAdjBinItems ? part % NB : -1, G, gN, gF, gS, gL)) {
-          continue;
-        }
-      }
-      add_chain(adj + s, gN, gF, gS, gL, NEAR, g);
-    }
-  } else if (!NEAR) {
+  if (!NEAR) {
     // far pairs: the stripe only.  Each thread fetches its next pair's record (the first 16 bytes
     // of the pair record) and chain factors while the current pair's table loads are in flight.
     struct FarItem {
@@ -724,7 +552,7 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
       x = y;
     }
   }
-  for (int it = t; !DENSE && NEAR && it < tot * NI; it += NT) {
+  for (int it = t; NEAR && it < tot * NI; it += NT) {
     const int i = it / NI, part = it - i * NI;
     while (i >= c_cum[cur + 1]) ++cur;
     const int s = c_beg[cur] + (i - c_cum[cur]);
@@ -756,6 +584,322 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
     double v = 0.0;
     for (int w = 0; w < NT / kWave; ++w) v += red[t][w];
     partial[(int64_t)r * kNGrad + t] = v;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Stage 2 with a waveform cotangent (lsim_backward_wfs)
+// ------------------------------------------------------------------------------------------
+//
+// G(o) = gw[r][o] + the self-trigger's step function: one value per output tick, so the window
+// sums are plain dot products, and they take the forward's access pattern with reads in place of
+// stores.  A whole wave works on one (segment, row) pair: lane l holds the padded window samples
+// p = l + 63 c (c < 3) of the pair (output tick o = S - 2 + p), G(o) comes from LDS and the next
+// sample (the f window) from lane l + 1 by a DPP shift; lane 63 only supplies that next sample.
+// The window sums of a pair enter the gradients linearly, so every lane scales its own share by
+// the pair's chain factors and adds it to its 7 partial gradients: no reduction per pair, one
+// per row at the end (fixed order: deterministic).
+//   Stripes (the raw-window stripe of every pixel offset but the centre, |a|, |b| <= nn): blocks
+//     of 64 pairs whose factors the lanes compute in parallel and stage in LDS; the wave then reads
+//     them back by broadcast, 4 pairs' window loads in flight at a time.  The boundary-correction
+//     terms (no window) are added by the lane that set the pair up.
+//   Diffusion bins landing on the row (the 3x3 cells; side rows only for segments near the pixel
+//     edge): one (segment, row) item per wave at a time, LUT cell by LUT cell from the interleaved
+//     template windows (T_{m-1}, T_m, T_{m+1}) of TM3: V = la T_{m-1} + lb T_m + lc T_{m+1} for
+//     n_e, f, sigma_T, and dV/dlambda = da (T_{m-1} - T_m) + dc (T_{m+1} - T_m) for lambda (the
+//     three Lagrange derivatives sum to zero, so the differences lose no precision).
+
+constexpr int kGOff = 192;                       // LDS index of output tick 0
+constexpr int kGLen = kMaxOut + 2 * kGOff + 64;  // ticks -kGOff .. kMaxOut + kGOff + 63
+
+// One stripe pair as staged for the wave (64 bytes).
+struct __align__(16) DenseStripe {
+  int S;             // window start, clamped to 2 when the window misses the row (factors 0 then)
+  int off;           // byte offset of the pair's padded raw-window row in TNP
+  float c1, c0, ne;  // 1 - f, f, charge (0 for padding and for windows that miss the row)
+  int pad;
+  double fN[4], fF;
+};
+static_assert(sizeof(DenseStripe) == 64, "DenseStripe layout");
+
+// G at a correction tick (those are kept for 0 <= o <= nout - 2); Gw holds 0 at tick 0 (a window
+// tick is kept from 1 on), G0 holds G(0).
+__device__ __forceinline__ double g_corr(const float* Gw, float G0, int o, int nout) {
+  return (o >= 0 && o <= nout - 2) ? (double)(o == 0 ? G0 : Gw[kGOff + o]) : 0.0;
+}
+
+// The window of a pair meets the output ticks 1 .. nout - 1 (window ticks S - 1 .. S + L - 1)
+__device__ __forceinline__ bool window_live(int S, int L, int nout) { return S >= 2 - L && S <= nout; }
+
+template <int NB, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void k_rows_adj_dense(Dev d, Tables tb, const SegRec* __restrict__ rec,
+                                                                  const PairRec* __restrict__ prs,
+                                                                  const AdjRec* __restrict__ adj,
+                                                                  const int2* __restrict__ rng,
+                                                                  const int* __restrict__ row_cell,
+                                                                  const int* __restrict__ n_rows_dev, int64_t max_rows,
+                                                                  const Pieces* __restrict__ pieces,
+                                                                  const float* __restrict__ gw,
+                                                                  double* __restrict__ partial) {
+  constexpr int NT = WAVES * kWave;
+  constexpr int SYM = (NB - 1) / 2;
+  __shared__ float Gw[kGLen];
+  __shared__ DenseStripe stage[WAVES][kWave];
+  __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
+  __shared__ int ncls[3];
+  __shared__ int cell_scratch[3 * kMaxCells];
+  __shared__ double red[kNGrad][WAVES];
+  __shared__ float sG0;
+  __shared__ int s_any;
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1), wid = t / kWave;
+  const int P0 = *n_rows_dev;
+  const int P = P0 < max_rows ? P0 : (int)max_rows;
+  double* out_near = partial + max_rows * kNGrad;  // the second half of the partials: zero here
+  if ((int64_t)blockIdx.x >= P) {
+    if (blockIdx.x < max_rows && t < kNGrad) {
+      partial[(int64_t)blockIdx.x * kNGrad + t] = 0.0;
+      out_near[(int64_t)blockIdx.x * kNGrad + t] = 0.0;
+    }
+    return;
+  }
+  const int xcd = (int)(blockIdx.x % kXcd);
+  const int q8 = P / kXcd, rem = P % kXcd;
+  const int r = xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
+  const int nout = d.nout, L = d.L;
+  // G over the padded tick range: window ticks 1 .. nout - 1, zero elsewhere
+  if (t == 0) s_any = 0;
+  __syncthreads();
+  {
+    const Pieces* pr = pieces + r;
+    const float* gr = gw + (int64_t)r * nout;
+    const int nf = pr->n;
+    bool any = false;
+    for (int i = t; i < kGLen; i += NT) {
+      const int o = i - kGOff;
+      float v = 0.f;
+      if (o >= 0 && o < nout) {
+        float cf = 0.f;
+        for (int k = nf - 1; k >= 0; --k)
+          if (pr->x[k] >= o) cf = pr->c[k];
+        v = gr[o] + cf;
+        any = any || v != 0.f;
+        if (o == 0) sG0 = v;
+      }
+      Gw[i] = (o >= 1 && o < nout) ? v : 0.f;
+    }
+    if (any) s_any = 1;
+  }
+  __syncthreads();
+  if (!s_any) {  // no cotangent on this row
+    if (t < kNGrad) {
+      partial[(int64_t)r * kNGrad + t] = 0.0;
+      out_near[(int64_t)r * kNGrad + t] = 0.0;
+    }
+    return;
+  }
+  const float G0 = sG0;
+  // cells: far (stripes), side (stripes and bins), centre (bins)
+  list_cells<NT>(
+      d, rng, row_cell[r],
+      [](int a, int b) {
+        const bool near = a >= -1 && a <= 1 && b >= -1 && b <= 1;
+        return !near ? 0 : ((a == 0 && b == 0) ? 2 : 1);
+      },
+      [](int b, int e) { return e - b; }, cell_scratch, c_beg, c_cum, c_ab, ncls);
+  double g[kNGrad];
+#pragma unroll
+  for (int k = 0; k < kNGrad; ++k) g[k] = 0.0;
+  const int nsb = d.nsb, lxly = d.lx * d.ly;
+  const int n_stripe_cells = d.nn >= 1 ? ncls[1] : ncls[0];
+  const int nstripe = c_cum[n_stripe_cells];
+  const __amdgpu_buffer_rsrc_t tnp =
+      __builtin_amdgcn_make_buffer_rsrc((void*)tb.TNP, 0, d.lx * d.ly * kWinPad * 4, kBufDesc3);
+  const float gmask = lane < kChunk ? 1.f : 0.f;  // lane 63 only supplies the next sample
+  // ---- stripes ----
+  for (int b0 = wid * kWave; b0 < nstripe; b0 += WAVES * kWave) {
+    const int p = b0 + lane;
+    DenseStripe z;
+    z.S = 2;
+    z.off = 0;
+    z.c1 = z.c0 = z.ne = 0.f;
+    z.pad = 0;
+    z.fN[0] = z.fN[1] = z.fN[2] = z.fN[3] = z.fF = 0.0;
+    if (p < nstripe) {
+      int lo = 0, hi = n_stripe_cells - 1;  // the cell holding p: last c with c_cum[c] <= p
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (c_cum[mid] <= p) lo = mid;
+        else hi = mid - 1;
+      }
+      const int sg = c_beg[lo] + (p - c_cum[lo]);
+      const int ab = c_ab[lo];
+      const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+      const PairRec R = prs[sg];
+      if (R.ne != 0.f) {
+        const int S = R.S;
+        const double f = (double)R.f, ne = (double)R.ne;
+        const int ux = (R.flags >> 8) & 255, uy = (R.flags >> 16) & 255;
+        const int sidx = offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb);
+        const float2 D = tb.DN[(int64_t)(d.nt - L - S) * lxly + sidx];  // tick-major, k0 = nt - L - S
+        const double corr = (double)D.x - f * (double)D.y, dy = (double)D.y;
+        const double gc0 = g_corr(Gw, G0, S - 1, nout), gc1 = g_corr(Gw, G0, S - 2, nout);
+        const double gc = (1.0 - f) * gc0 + f * gc1;
+        const AdjRec* A = adj + sg;
+        const double fN0 = A->fN[0], fN1 = A->fN[1], fN2 = A->fN[2], fN3 = A->fN[3], fF = A->fF;
+        // the correction terms of the pair (no window)
+        const double gNc = gc * corr, gFc = ne * ((gc1 - gc0) * corr - gc * dy);
+        g[0] += gNc * fN0;
+        g[1] += gNc * fN1;
+        g[2] += gNc * fN2 + gFc * fF;
+        g[3] += gNc * fN3;
+        if (window_live(S, L, nout)) {
+          z.S = S;
+          z.off = sidx * kWinPad * 4;
+          z.c1 = (float)(1.0 - f);
+          z.c0 = R.f;
+          z.ne = R.ne;
+          z.fN[0] = fN0;
+          z.fN[1] = fN1;
+          z.fN[2] = fN2;
+          z.fN[3] = fN3;
+          z.fF = fF;
+        }
+      }
+    }
+    stage[wid][lane] = z;
+    __builtin_amdgcn_wave_barrier();
+    const int n = __builtin_amdgcn_readfirstlane(nstripe - b0 < kWave ? nstripe - b0 : kWave);
+    for (int i = 0; i < n; i += 4) {
+      // pairs i .. i + 3 (entries past n are padding: factors 0, a window at S = 2)
+      DenseStripe q[4];
+      float x[4][3];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        q[k] = stage[wid][i + k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          x[k][c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tnp, lane * 4 + c * 4 * kChunk + q[k].off, 0, 0));
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float u1 = 0.f, u0 = 0.f;
+        const float* gp = Gw + kGOff + q[k].S - 2 + lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float gv = gmask * gp[c * kChunk];
+          const float xn = lane_next0(x[k][c]);
+          u1 += gv * x[k][c];
+          u0 += gv * xn;
+        }
+        const double aa = (double)(q[k].c1 * u1 + q[k].c0 * u0), bb = (double)(q[k].ne * (u0 - u1));
+        g[0] += q[k].fN[0] * aa;
+        g[1] += q[k].fN[1] * aa;
+        g[2] += q[k].fN[2] * aa + q[k].fF * bb;
+        g[3] += q[k].fN[3] * aa;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // the stage is rewritten by the next block
+  }
+  // ---- diffusion bins of the 3x3 cells ----
+  const int kb0 = c_cum[ncls[0]], kb1 = c_cum[ncls[2]];
+  int cur = ncls[0];
+  const double lead = lane == 0 ? 1.0 : 0.0;  // terms without a window: once per item
+  for (int k = kb0 + wid; k < kb1; k += WAVES) {
+    while (k >= c_cum[cur + 1]) ++cur;
+    const int s = __builtin_amdgcn_readfirstlane(c_beg[cur] + (k - c_cum[cur]));
+    const int ab = __builtin_amdgcn_readfirstlane(c_ab[cur]);
+    const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
+    const SegRec R = rec[s];
+    if (R.ne == 0.f) continue;
+    const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
+    if (!bins_reach<NB>(ux, a, nsb) || !bins_reach<NB>(uy, b, nsb)) continue;
+    const AdjRec* A = adj + s;
+    const int S = d.nt - L - R.k0;
+    const double f = (double)R.f, ne = (double)R.ne;
+    const double gc0 = g_corr(Gw, G0, S - 1, nout), gc1 = g_corr(Gw, G0, S - 2, nout);
+    const double gc = (1.0 - f) * gc0 + f * gc1;
+    const bool live = window_live(S, L, nout);
+    const float* gp = Gw + kGOff + (live ? S : 2) - 2 + lane;
+    float gv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gv[c] = live ? gmask * gp[c * kChunk] : 0.f;
+    bool oky[NB];
+    int cyv[NB];
+    float WY[NB], DWY[NB];
+    axis_cells<NB>(uy, b, nsb, R.wy, A->dwy, oky, cyv, WY, DWY);
+    const float la = R.la, lb = R.lb, lc = R.lc;
+    const float da = (float)A->dl[0], dc = (float)A->dl[2];
+    const int cm = d.cm;
+    double gN = 0.0, gF = 0.0, gS = 0.0, gL = 0.0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int vi = ux + i - SYM;
+      const int di = vi < 0 ? -1 : (vi >= nsb ? 1 : 0);
+      if (di != a) continue;
+      const int cx = offset_bin(vi - di * nsb, 0, nsb);
+      // x-bins on the same LUT cell are merged into the first of them
+      float WXi = 0.f, DWXi = 0.f;
+      bool first = true;
+#pragma unroll
+      for (int kk = 0; kk < NB; ++kk) {
+        const int v = ux + kk - SYM;
+        const int dv = v < 0 ? -1 : (v >= nsb ? 1 : 0);
+        if (dv != a || offset_bin(v - dv * nsb, 0, nsb) != cx) continue;
+        if (kk < i) first = false;
+        WXi += R.wx[kk];
+        DWXi += A->dwx[kk];
+      }
+      if (!first) continue;
+      // the y cells of this x cell: all window loads first, then the sums
+      float3 q[NB][3];
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        const int cy = oky[jj] ? cyv[jj] : cyv[0];
+        const float3* T3 = tb.TM3 + (((int64_t)R.idx * cm + cx) * cm + cy) * kWinPad + lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[jj][c] = T3[c * kChunk];
+      }
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        if (!oky[jj]) continue;
+        const int64_t cell = ((int64_t)R.idx * cm + cx) * cm + cyv[jj];
+        float u1 = 0.f, u0 = 0.f, ul1 = 0.f, ul0 = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float V = q[jj][c].y * lb + q[jj][c].x * la + q[jj][c].z * lc;
+          const float VL = da * (q[jj][c].x - q[jj][c].y) + dc * (q[jj][c].z - q[jj][c].y);
+          const float Vn = lane_next0(V), VLn = lane_next0(VL);
+          u1 += gv[c] * V;
+          u0 += gv[c] * Vn;
+          ul1 += gv[c] * VL;
+          ul0 += gv[c] * VLn;
+        }
+        const float2 D = tb.DM[(int64_t)R.k0 * d.nm * cm * cm + cell];  // tick-major
+        const double corr = lead * ((double)D.x - f * (double)D.y), dy = lead * (double)D.y;
+        const double wxy = (double)WXi * (double)WY[jj];
+        const double dwxy = (double)DWXi * (double)WY[jj] + (double)WXi * (double)DWY[jj];
+        const double gn = (1.0 - f) * (double)u1 + f * (double)u0 + gc * corr;
+        gN += gn * wxy;
+        gS += gn * ne * dwxy;
+        gF += ne * wxy * ((double)u0 - (double)u1 + (gc1 - gc0) * corr - gc * dy);
+        gL += ne * wxy * ((1.0 - f) * (double)ul1 + f * (double)ul0);
+      }
+    }
+    add_chain(A, gN, gF, gS, gL, true, g);
+  }
+  // fixed-order block reduction of the lanes' partial gradients
+#pragma unroll
+  for (int k = 0; k < kNGrad; ++k) {
+    const double v = wave_sum(g[k]);
+    if (lane == 0) red[k][wid] = v;
+  }
+  __syncthreads();
+  if (t < kNGrad) {
+    double v = 0.0;
+    for (int w = 0; w < WAVES; ++w) v += red[t][w];
+    partial[(int64_t)r * kNGrad + t] = v;
+    out_near[(int64_t)r * kNGrad + t] = 0.0;
   }
 }
 

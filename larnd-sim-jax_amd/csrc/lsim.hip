@@ -839,27 +839,31 @@ static int backward_impl(lsim_ctx* c, const lsim_params* p, const lsim_segments*
       hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, in, keys, vals, n, adj);
     const dim3 grid((unsigned)max_rows), block(kAdjWaves * kWave);
     double* pnear = partial + max_rows * kNGrad;
-#define LSIM_ADJ_LAUNCH(NBV, DN)                                                                                    \
-  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false, DN>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,   \
-                     w.row_cell, w.n_rows, max_rows, pieces, d_wfs, partial);                                       \
-  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true, DN>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,    \
-                     w.row_cell, w.n_rows, max_rows, pieces, d_wfs, pnear)
+#define LSIM_ADJ_LAUNCH(NBV)                                                                                        \
+  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,       \
+                     w.row_cell, w.n_rows, max_rows, pieces, partial);                                              \
+  hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,        \
+                     w.row_cell, w.n_rows, max_rows, pieces, pnear)
+#define LSIM_ADJ_DENSE(NBV)                                                                                       \
+  hipLaunchKernelGGL((k_rows_adj_dense<NBV, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,       \
+                     w.row_cell, w.n_rows, max_rows, pieces, d_wfs, partial)
     if (d_wfs) {
       if (d.nb == 5) {
-        LSIM_ADJ_LAUNCH(5, true);
+        LSIM_ADJ_DENSE(5);
       } else if (d.nb == 3) {
-        LSIM_ADJ_LAUNCH(3, true);
+        LSIM_ADJ_DENSE(3);
       } else {
-        LSIM_ADJ_LAUNCH(1, true);
+        LSIM_ADJ_DENSE(1);
       }
     } else if (d.nb == 5) {
-      LSIM_ADJ_LAUNCH(5, false);
+      LSIM_ADJ_LAUNCH(5);
     } else if (d.nb == 3) {
-      LSIM_ADJ_LAUNCH(3, false);
+      LSIM_ADJ_LAUNCH(3);
     } else {
-      LSIM_ADJ_LAUNCH(1, false);
+      LSIM_ADJ_LAUNCH(1);
     }
 #undef LSIM_ADJ_LAUNCH
+#undef LSIM_ADJ_DENSE
   }
   hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(kReduceThreads), 0, st, d, partial, 2 * max_rows, dz, fwd->n_rows,
                      max_rows, d_params);
