@@ -62,9 +62,6 @@ def onchip_per_segment(n: int = 4, nb: int = 5, L: int = 150) -> dict:
     return dict(scatter_adds=scatter, lut_gathers=gathers, lds_bytes=8 * scatter + 4 * gathers)
 
 
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
-
-
 def copy_gbs(dev, nbytes: int = 1 << 30, reps: int = 5) -> float:
     """Achievable HBM bandwidth: a plain device copy of nbytes (read + write counted)."""
     import torch
@@ -263,6 +260,77 @@ def record_traffic(d, kernel_ms):
                bytes_per_launch=fk * 1024 / ff + wk * 1024 / wf, kernel_ms=kernel_ms,
                source_stamp=kernel_source_stamp())
     with open(TRAFFIC_JSON, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    return rec
+
+
+COUNTERS_JSON = os.path.join(ROOT, "profiles", "kernel_counters_128_events.json")
+COUNTER_KERNELS = ("k_rows_fwd", "k_center", "k_records")
+
+
+def _counter_means(d):
+    """{kernel: {counter: mean per dispatch}} over the rocprofv3 counter CSVs under d, the first
+    dispatch of each kernel excluded (cold caches), counter instances summed per dispatch."""
+    import csv
+    import glob
+
+    acc = {}
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                name = next((k for k in COUNTER_KERNELS if k in row["Kernel_Name"]), None)
+                if name is None:
+                    continue
+                per = acc.setdefault(name, {}).setdefault(row["Counter_Name"], {})
+                k = (path, int(row["Dispatch_Id"]))
+                per[k] = per.get(k, 0.0) + float(row["Counter_Value"])
+    out = {}
+    for name, cs in acc.items():
+        out[name] = {}
+        for c, v in cs.items():
+            keys = sorted(v)
+            vals = [v[k] for k in keys[1:]] or [v[k] for k in keys]
+            out[name][c] = sum(vals) / len(vals)
+    return out
+
+
+def record_counters(d, cu_num=256, xcds=8):
+    """Derived issue, LDS and L2 figures of the forward kernels from rocprofv3 --pmc passes of
+    `bench.py --profile-batch` under d, stamped with the kernel sources, into COUNTERS_JSON.
+    GRBM_GUI_ACTIVE is summed over the XCD instances, so the kernel's cycles are that sum / xcds;
+    a wave64 VALU instruction occupies a SIMD for 4 cycles (16 lanes)."""
+    raw = _counter_means(d)
+    derived = {}
+    for name, c in raw.items():
+        cyc = c.get("GRBM_GUI_ACTIVE", 0.0) / xcds
+        r = {}
+        if cyc > 0 and "SQ_ACTIVE_INST_VALU" in c:
+            r["valu_busy"] = c["SQ_ACTIVE_INST_VALU"] / cu_num / cyc
+        if cyc > 0 and "SQ_INSTS_VALU" in c:
+            r["valu_issue_frac"] = c["SQ_INSTS_VALU"] * 4 / (cu_num * 4 * cyc)
+        if c.get("SQ_ACTIVE_INST_VALU"):
+            r["valu_lane_util"] = c.get("SQ_THREAD_CYCLES_VALU", 0.0) / (c["SQ_ACTIVE_INST_VALU"] * 64)
+        if c.get("SQ_LDS_IDX_ACTIVE"):
+            free = max(c["SQ_LDS_IDX_ACTIVE"] - c["SQ_LDS_BANK_CONFLICT"], 1.0)
+            r["lds_bank_conflict_rate"] = c["SQ_LDS_BANK_CONFLICT"] / free
+        if cyc > 0 and "SQ_LDS_IDX_ACTIVE" in c:
+            r["lds_busy"] = c["SQ_LDS_IDX_ACTIVE"] / cu_num / cyc
+        if c.get("TCC_REQ"):
+            r["l2_hit_rate"] = c["TCC_HIT"] / c["TCC_REQ"]
+        if c.get("SQ_WAVE_CYCLES"):
+            r["wave_wait_frac"] = c.get("SQ_WAIT_ANY", 0.0) / c["SQ_WAVE_CYCLES"]
+        r["kernel_cycles_per_xcd"] = cyc
+        derived[name] = r
+    rec = dict(workload="128 events, 9 160 058 segments, 457 979 pixel rows (`bench.py --profile-batch`)",
+               derived=derived, raw=raw, source_stamp=kernel_source_stamp(),
+               definitions=dict(valu_busy="SQ_ACTIVE_INST_VALU / CU_NUM / kernel cycles",
+                                valu_issue_frac="VALU instructions x 4 cycles / (4 SIMDs x CUs x kernel cycles)",
+                                valu_lane_util="SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64)",
+                                lds_bank_conflict_rate="SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - "
+                                                       "SQ_LDS_BANK_CONFLICT)",
+                                lds_busy="SQ_LDS_IDX_ACTIVE / CU_NUM / kernel cycles",
+                                l2_hit_rate="TCC_HIT / TCC_REQ", wave_wait_frac="SQ_WAIT_ANY / SQ_WAVE_CYCLES"))
+    with open(COUNTERS_JSON, "w") as fh:
         json.dump(rec, fh, indent=1)
     return rec
 
@@ -502,6 +570,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--waves", type=int, default=0, help="waves per row workgroup (0 = autotune)")
+    ap.add_argument("--row-tile", type=int, default=16, help="cell tile of the row order (with --waves)")
     ap.add_argument("--row-cost", type=int, default=256,
                     help="per-row cost (pairs) of the XCD band cuts when --waves is given (else autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -528,9 +597,15 @@ def main():
                     help="write the waveform kernel's HBM traffic from rocprofv3 counter passes under DIR "
                          "(fetch/, write/, copy_fetch/, copy_write/) into profiles/, stamped; needs --kernel-ms")
     ap.add_argument("--kernel-ms", type=float, default=None)
+    ap.add_argument("--record-counters", metavar="DIR",
+                    help="write issue, LDS and L2 figures of the forward kernels from rocprofv3 --pmc passes of "
+                         "--profile-batch under DIR into profiles/, stamped")
     a = ap.parse_args()
     if a.record_traffic:
         print(json.dumps(record_traffic(a.record_traffic, a.kernel_ms)))
+        return
+    if a.record_counters:
+        print(json.dumps(record_counters(a.record_counters)["derived"]))
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events would shard across "
@@ -596,10 +671,12 @@ def main():
     if a.waves:
         S.set_row_waves(a.waves)
         S.set_row_cost(a.row_cost)
-        waves, row_cost, tune = a.waves, a.row_cost, {}
+        S.set_row_tile(a.row_tile)
+        waves, row_cost, row_tile, tune = a.waves, a.row_cost, a.row_tile, {}
     else:
-        waves, row_cost, tune = S.autotune_rows(brun)
-    log(f"[bench] waves per row workgroup: {waves}, row cost {row_cost} pairs (tuning ms: {tune})")
+        waves, row_cost, row_tile, tune = S.autotune_rows(brun)
+    log(f"[bench] waves per row workgroup: {waves}, row cost {row_cost} pairs, row tile {row_tile} "
+        f"(tuning ms: {tune})")
     g = torch.cuda.CUDAGraph()
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
@@ -898,12 +975,16 @@ def main():
 
     achievable = copy_gbs(dev)
     log(f"[bench] device copy {achievable:.0f} GB/s")
+    # issue, LDS and L2 figures measured with rocprofv3 counters (valid for the sources they measured)
+    onchip = {"source": os.path.relpath(COUNTERS_JSON, ROOT), "measured": None}
+    if os.path.exists(COUNTERS_JSON):
+        with open(COUNTERS_JSON) as f:
+            crec = json.load(f)
+        onchip["stale"] = crec.get("source_stamp") != kernel_source_stamp()
+        if not onchip["stale"]:
+            onchip["measured"] = crec.get("derived", {}).get("k_rows_fwd")
     oc = onchip_per_segment(int(params.number_pix_neighbors), int(params.nb_tran_diff_bins), int(params.signal_length))
-    onchip = dict(per_segment=oc, lds_bytes=oc["lds_bytes"] * NB, lds_gbs=oc["lds_bytes"] * NB / (k_avg * 1e-3) / 1e9,
-                  lds_peak_gbs=LDS_PEAK_GBS, lds_frac=oc["lds_bytes"] * NB / (k_avg * 1e-3) / 1e9 / LDS_PEAK_GBS,
-                  note="the algorithm's scatter-adds and LUT gathers counted as LDS bytes over the waveform "
-                       "kernel's time (an upper bound: the kernel merges each (1-f, f) split into one "
-                       "read-modify-write); measured LDS and issue counters are in profiles/")
+    onchip["algorithm_per_segment"] = oc
 
     # HBM traffic per launch from counter passes, valid only for the kernel sources they measured
     traffic, stamp = None, None
@@ -933,7 +1014,7 @@ def main():
                                "hipGraph replay per step, module0 geometry, 45x45x1950 LUT, 4 neighbour pixels, "
                                "5x5 diffusion bins, noise off",
                    "segments": NB, "events": 128, "pixel_rows": PB, "row_waves": waves, "row_cost": row_cost,
-                   "tuning_ms": tune},
+                   "row_tile": row_tile, "tuning_ms": tune},
         "roofline": {"bound": "hbm", "kernel": "k_rows_fwd", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_stamp": stamp, "algorithmic_bytes": bytes_fwd, "kernel_ms": k_avg,

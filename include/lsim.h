@@ -193,9 +193,15 @@ int lsim_timing_read(double *total_ms, int64_t *launches);
 int lsim_set_row_waves(int waves);
 
 /* Tuning: the fixed cost of one pixel row in units of segment-row pairs (default 256), used to cut
- * the rows into per-XCD bands of equal estimated work for the persistent waveform kernel.  Only
+ * the rows into per-XCD bands of equal estimated work for the waveform kernel.  Only
  * the schedule changes: results are the same for every value. */
 int lsim_set_row_cost(int pairs);
+
+/* Scheduling knob of the waveform kernel, no counterpart in the reference (like the two above): the
+ * pixel rows are processed tile by tile, `cells` x `cells` pixel cells of one event plane per tile
+ * (default 16; 1 = plain row order), so that the rows running together on one XCD share their
+ * segments in its L2.  Changes only the order rows are computed in, never their values. */
+int lsim_set_row_tile(int cells);
 
 /* Backward workspace bytes for n_seg segments and max_rows rows. */
 int lsim_backward_query(lsim_ctx *ctx, int64_t n_seg, int64_t max_rows, size_t *bwd_ws_bytes);
@@ -242,6 +248,33 @@ int lsim_rbf_sums(const float *x, int64_t nx, const int32_t *lo, const int32_t *
 int lsim_rbf_sums_grid(const float *x, const int32_t *x_event, int64_t nx, const float *y, const float *c,
                        const int64_t *y_key, int64_t ny, const float *grid, int32_t dims, float sigma, double *out,
                        void *stream);
+
+/* Parametrized current model (simulate_parametrized, src/larndsim/sim_jax.py:339-372), step 1: the
+ * electrons of the drifted segments and their pixel pairs.  mode 0: the transverse bins of
+ * apply_tran_diff (detsim_jax.py:356-373) on bin_edges [n_edges] float64 (device), n_edges - 1 bins per
+ * axis, (n_edges - 1)^2 electrons per segment; mode 1: one electron per segment spread by the caller's
+ * standard-normal draws normals [n][3] float32 (mc_diff, detsim_jax.py:377-399; z is spread only when
+ * diffusion == 0, i.e. the current model carries none).  Outputs, E = n * electrons per segment, in
+ * (segment, sub-electron) order: ex, ey (cm), et0 (drift time to the anode, us), ene (e-), esig (the
+ * longitudinal spread in time, us) float64 [E], eplane int32 [E], and pair_pids int64 [E][(2n+1)^2], the
+ * pixel ids around each electron's pixel in (a, b) row-major order, -1 outside the plane
+ * (get_pixels, detsim_jax.py:483-496).  Float64 throughout. */
+int lsim_param_electrons(lsim_ctx *c, const lsim_params *p, const lsim_segments *seg, int32_t mode, int32_t diffusion,
+                         const float *normals, const double *bin_edges, int32_t n_edges, double *ex, double *ey,
+                         double *et0, double *ene, double *esig, int32_t *eplane, int64_t *pair_pids, void *stream);
+
+/* Step 2: the pixel waveforms wfs [n_rows][n_ticks - 1] float32 of the sorted pixel set pixel_ids [n_rows]
+ * (current_mc, detsim_jax.py:623-639, with current_model / current_model_diff :544-622, and
+ * accumulate_signals_parametrized :209-228): row r adds, for the pairs pair_order[row_offsets[r] ..
+ * row_offsets[r + 1]) (pair index = electron * (2n+1)^2 + offset, in ascending order within a row), the
+ * electron's charge times the induced current at the int(5 / t_sampling) + 1 sample times of
+ * linspace(0, 5) us, at waveform ticks t0_tick - samples + k (ticks outside the waveform are dropped, as
+ * the reference's column 0 is).  diffusion != 0: the exponentially modified Gaussian tails of
+ * current_model_diff with the electron's spread.  Rows with a negative id stay zero.  Float64 sums in a
+ * fixed order (deterministic). */
+int lsim_param_rows(lsim_ctx *c, const lsim_params *p, const double *ex, const double *ey, const double *et0,
+                    const double *ene, const double *esig, const int64_t *pixel_ids, int64_t n_rows,
+                    const int32_t *pair_order, const int32_t *row_offsets, int32_t diffusion, float *wfs, void *stream);
 
 /* The distance beyond which lsim_rbf_sums_grid leaves pairs out (1.01 sqrt(212) sigma); callers size
  * the grid's cells from it (any cell size is correct; about a third of it culls well). */

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "lsim.hip")
 OUT = os.path.join(HERE, "larndsim_amd", "liblsim.so")
-HEADERS = [os.path.join(HERE, "csrc", h) for h in ("device.h", "scan_sort.h", "forward.h", "backward.h", "loss.h", "chop.h", "probfee.h", "sdtw.h")]
+HEADERS = [os.path.join(HERE, "csrc", h) for h in ("device.h", "scan_sort.h", "forward.h", "backward.h", "loss.h", "chop.h", "probfee.h", "sdtw.h", "param.h")]
 
 
 def hipcc() -> str:

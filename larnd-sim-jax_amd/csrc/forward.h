@@ -284,11 +284,13 @@ struct RowOut {
   int64_t* pixel_ids;  // [max_rows]
 };
 
+// NaN passes through both clamps, as through jnp.maximum / jnp.minimum (a waveform poisoned by an
+// overflowing current keeps its NaN charge, as in the reference).
 __device__ __forceinline__ double digitize(const Dev& d, double q) {
   double x = q * d.gain + d.vped - d.vcm;
-  x = x > 0 ? x : 0.0;
+  x = (x > 0 || x != x) ? x : 0.0;
   double y = x * d.adcc / (d.vref - d.vcm);
-  return y < d.adcc ? y : d.adcc;
+  return (y < d.adcc || y != y) ? y : d.adcc;
 }
 
 // JAX's derivative of digitize (ties of max/min split the gradient evenly).
@@ -858,23 +860,18 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
 // the row's (2e+1)^2 neighbourhood cells plus a fixed per-row cost (row set-up, sum, cumulative
 // charge and self-trigger) in the same units.
 
-// Processing order of the pixel rows: by tile of kRowTile x kRowTile cells of one event plane, in
-// row order within a tile (a stable sort on the tile id).  Consecutive workgroups of an XCD then
-// work on neighbouring pixels, whose neighbourhoods share most of their segments, so the segment
+// Processing order of the pixel rows: by tile of `tile` x `tile` cells of one event plane, in row
+// order within a tile (a stable sort on the tile id).  Consecutive workgroups of an XCD then work
+// on neighbouring pixels, whose neighbourhoods share most of their segments, so the segment
 // records and corrections a band reads stay in that XCD's L2.  The order changes which workgroup
-// computes a row and when, never what it computes.
-#ifndef LSIM_ROW_TILE
-#define LSIM_ROW_TILE 16
-#endif
-constexpr int kRowTile = LSIM_ROW_TILE;
-
-inline int64_t row_tiles(const Dev& d) {
-  const int64_t ntx = (d.gx + kRowTile - 1) / kRowTile, nty = (d.gy + kRowTile - 1) / kRowTile;
+// computes a row and when, never what it computes (tile 1: plain row order).
+inline int64_t row_tiles(const Dev& d, int tile) {
+  const int64_t ntx = (d.gx + tile - 1) / tile, nty = (d.gy + tile - 1) / tile;
   return d.n_cells / ((int64_t)d.gx * d.gy) * ntx * nty;
 }
 
 // Sort keys of the rows: the tile id, or `past` for the rows P..max_rows-1 (after every tile).
-__global__ void k_tile_keys(Dev d, const int* __restrict__ row_cell, const int* __restrict__ n_rows_dev,
+__global__ void k_tile_keys(Dev d, int tile, const int* __restrict__ row_cell, const int* __restrict__ n_rows_dev,
                             int64_t max_rows, uint32_t past, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= max_rows) return;
@@ -886,8 +883,8 @@ __global__ void k_tile_keys(Dev d, const int* __restrict__ row_cell, const int* 
   }
   const int cell = row_cell[r];
   const int cx = cell % d.gx, cy = (cell / d.gx) % d.gy, evp = cell / (d.gx * d.gy);
-  const uint32_t ntx = (uint32_t)((d.gx + kRowTile - 1) / kRowTile), nty = (uint32_t)((d.gy + kRowTile - 1) / kRowTile);
-  keys[r] = ((uint32_t)evp * nty + (uint32_t)(cy / kRowTile)) * ntx + (uint32_t)(cx / kRowTile);
+  const uint32_t ntx = (uint32_t)((d.gx + tile - 1) / tile), nty = (uint32_t)((d.gy + tile - 1) / tile);
+  keys[r] = ((uint32_t)evp * nty + (uint32_t)(cy / tile)) * ntx + (uint32_t)(cx / tile);
 }
 
 // Estimated work of every row, in processing order (work[s] for the row order[s]): one thread per

@@ -12,6 +12,7 @@
 #include "backward.h"
 #include "chop.h"
 #include "loss.h"
+#include "param.h"
 #include "probfee.h"
 #include "sdtw.h"
 
@@ -256,8 +257,11 @@ int g_row_cost = 256;
 // is kXcd times this, so at most a third of its workgroups find no row).
 int64_t band_slots(int64_t max_rows) { return (3 * max_rows + 2 * kXcd - 1) / (2 * kXcd); }
 
+// Side of the square cell tiles of the rows' processing order (lsim_set_row_tile).
+int g_row_tile = 16;
+
 // Bits of the tile-order sort key: every tile id and the key of the rows past P.
-int tile_key_bits(const Dev& d) { return bits_for(row_tiles(d) + 1); }
+int tile_key_bits(const Dev& d) { return bits_for(row_tiles(d, g_row_tile) + 1); }
 
 // The processing order of the rows (sorted by tile: k_tile_keys + radix sort); the buffer it lands
 // in depends only on the key width, like sorted_buffers.
@@ -270,7 +274,7 @@ const int* row_order(const Ws& w, const Dev& d) {
 int launch_row_bands(const Dev& d, const Ws& w, int64_t max_rows, hipStream_t st) {
   const int bits = tile_key_bits(d);
   const unsigned g = (unsigned)((max_rows + 255) / 256);
-  hipLaunchKernelGGL(k_tile_keys, dim3(g), dim3(256), 0, st, d, w.row_cell, w.n_rows, max_rows,
+  hipLaunchKernelGGL(k_tile_keys, dim3(g), dim3(256), 0, st, d, g_row_tile, w.row_cell, w.n_rows, max_rows,
                      (uint32_t)(((uint64_t)1 << bits) - 1), w.tk0, w.tv0);
   hipError_t err;
   radix_sort_pairs(w.tk0, w.tv0, w.tk1, w.tv1, max_rows, bits, w.sort_scratch, st, &err);
@@ -358,6 +362,12 @@ int lsim_set_row_waves(int waves) {
 int lsim_set_row_cost(int pairs) {
   if (pairs < 1 || pairs > (1 << 20)) return fail(LSIM_E_ARG, "row cost must be in [1, 2^20] pairs");
   g_row_cost = pairs;
+  return LSIM_OK;
+}
+
+int lsim_set_row_tile(int cells) {
+  if (cells < 1 || cells > 4096) return fail(LSIM_E_ARG, "row tile must be in [1, 4096] cells");
+  g_row_tile = cells;
   return LSIM_OK;
 }
 
@@ -896,6 +906,52 @@ int lsim_rbf_sums(const float* x, int64_t nx, const int32_t* lo, const int32_t* 
   const float inv = 1.f / (2.f * sigma * sigma);
   hipLaunchKernelGGL(k_rbf_sums<4>, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                      (const float4*)x, nx, lo, hi, (const float4*)y, (const float4*)c, inv, out);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+int lsim_param_electrons(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t mode, int32_t diffusion,
+                         const float* normals, const double* bin_edges, int32_t n_edges, double* ex, double* ey,
+                         double* et0, double* ene, double* esig, int32_t* eplane, int64_t* pair_pids, void* stream) {
+  if (!c || !p || !seg) return fail(LSIM_E_ARG, "NULL argument");
+  if (mode != 0 && mode != 1) return fail(LSIM_E_ARG, "mode must be 0 (transverse bins) or 1 (normal draws)");
+  if (mode == 0 && (n_edges < 2 || n_edges > kParamMaxEdges || !bin_edges))
+    return fail(LSIM_E_ARG, "transverse bins need 2 .. 17 bin edges");
+  if (mode == 1 && seg->n > 0 && !normals) return fail(LSIM_E_ARG, "normal draws are NULL");
+  Dev d;
+  int rc = make_dev(c, p, 1, d);
+  if (rc) return rc;
+  rc = check_segments(seg, d.recomb);
+  if (rc) return rc;
+  if (seg->n == 0) return LSIM_OK;
+  if (!ex || !ey || !et0 || !ene || !esig || !eplane || !pair_pids) return fail(LSIM_E_ARG, "NULL output");
+  const SegIn in = make_segin(seg);
+  ParamE e{ex, ey, et0, ene, esig, eplane};
+  hipLaunchKernelGGL(k_param_electrons, dim3((unsigned)((seg->n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, d, in,
+                     mode, diffusion, normals, bin_edges, n_edges, e, pair_pids);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
+int lsim_param_rows(lsim_ctx* c, const lsim_params* p, const double* ex, const double* ey, const double* et0,
+                    const double* ene, const double* esig, const int64_t* pixel_ids, int64_t n_rows,
+                    const int32_t* pair_order, const int32_t* row_offsets, int32_t diffusion, float* wfs, void* stream) {
+  if (!c || !p) return fail(LSIM_E_ARG, "NULL argument");
+  if (n_rows < 0 || n_rows >= ((int64_t)1 << 31)) return fail(LSIM_E_ARG, "n_rows out of range");
+  Dev d;
+  int rc = make_dev(c, p, 1, d);
+  if (rc) return rc;
+  const int T = (int)(5.0 / d.ts) + 1;  // linspace(0, 5, int(5 / t_sampling) + 1)
+  if (T < 2 || T > kParamMaxSamples) return fail(LSIM_E_ARG, "t_sampling must give 2 .. 64 current samples in 5 us");
+  if (d.nout > kMaxOut) return fail(LSIM_E_ARG, "too many output ticks");
+  if (n_rows == 0) return LSIM_OK;
+  if (!ex || !ey || !et0 || !ene || !esig || !pixel_ids || !pair_order || !row_offsets || !wfs)
+    return fail(LSIM_E_ARG, "NULL argument");
+  const int K = (2 * d.nn + 1) * (2 * d.nn + 1);
+  ParamE e{const_cast<double*>(ex), const_cast<double*>(ey), const_cast<double*>(et0), const_cast<double*>(ene),
+           const_cast<double*>(esig), nullptr};
+  hipLaunchKernelGGL(k_param_rows, dim3((unsigned)n_rows), dim3(kParamWaves * kWave), 0, (hipStream_t)stream, d, e,
+                     pixel_ids, pair_order, row_offsets, K, T, diffusion, wfs);
   HIP_TRY(hipGetLastError());
   return LSIM_OK;
 }

@@ -27,6 +27,11 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* sh, T& total) {
     T u = shfl_up_t(inc, d);
     if (lane >= d) inc += u;
   }
+  // the exclusive prefix is the previous lane's inclusive one (not inc - v: that would round, and
+  // a NaN or inf of this thread's own values would reach its prefix, which a cumulative sum's
+  // earlier entries never see)
+  T exc = shfl_up_t(inc, 1);
+  if (lane == 0) exc = 0;
   if (lane == kWave - 1) sh[wid] = inc;
   __syncthreads();
   const int nw = blockDim.x / kWave;
@@ -37,7 +42,7 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* sh, T& total) {
   }
   total = tot;
   __syncthreads();
-  return base + inc - v;
+  return base + exc;
 }
 
 // 1) per-tile sums

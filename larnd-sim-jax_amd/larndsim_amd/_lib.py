@@ -20,9 +20,11 @@ EXPORTS = (
     "lsim_forward_query", "lsim_forward_prepare", "lsim_forward_rows", "lsim_forward", "lsim_fee_query",
     "lsim_fee_forward", "lsim_fee_forward_noalloc", "lsim_negative_rows_query", "lsim_negative_rows",
     "lsim_backward_query", "lsim_backward", "lsim_backward_wfs", "lsim_set_row_waves", "lsim_set_row_cost",
+    "lsim_set_row_tile",
     "lsim_segment_records", "lsim_timing", "lsim_timing_read", "lsim_rbf_sums", "lsim_rbf_sums_grid",
     "lsim_rbf_cull_radius", "lsim_nearest", "lsim_chop_query", "lsim_chop_count", "lsim_chop_fill",
     "lsim_prob_fee_query", "lsim_prob_fee", "lsim_prob_fee_backward", "lsim_sdtw_query", "lsim_sdtw",
+    "lsim_param_electrons", "lsim_param_rows",
 )
 
 
@@ -111,6 +113,7 @@ def load(path: str = LIB_PATH):
                               vp, vp, vp, sz, vp, vp],
         "lsim_set_row_waves": [C.c_int],
         "lsim_set_row_cost": [C.c_int],
+        "lsim_set_row_tile": [C.c_int],
         "lsim_segment_records": [vp, i64, i32, vp, sz, vp, vp, vp, vp],
         "lsim_timing": [C.c_int],
         "lsim_timing_read": [C.POINTER(C.c_double), C.POINTER(i64)],
@@ -121,6 +124,9 @@ def load(path: str = LIB_PATH):
         "lsim_prob_fee": [vp, i64, i32, i32, i32, i32, C.c_float, C.c_float, C.c_float, C.c_float, vp, sz, vp, vp, vp],
         "lsim_prob_fee_backward": [i64, i32, i32, i32, i32, C.c_float, C.c_float, C.c_float, vp, sz, vp, vp, vp, vp,
                                    vp],
+        "lsim_param_electrons": [vp, C.POINTER(ParamsC), C.POINTER(Segments), i32, i32, vp, vp, i32, vp, vp, vp, vp,
+                                 vp, vp, vp, vp],
+        "lsim_param_rows": [vp, C.POINTER(ParamsC), vp, vp, vp, vp, vp, vp, i64, vp, vp, i32, vp, vp],
         "lsim_sdtw_query": [i64, i64, C.POINTER(sz)],
         "lsim_sdtw": [vp, i64, vp, i64, C.c_float, vp, sz, vp, vp, vp],
         "lsim_chop_query": [i64, C.POINTER(sz)],

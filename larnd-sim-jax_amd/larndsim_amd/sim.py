@@ -125,7 +125,9 @@ class ForwardState:
 
 
 class Simulator:
-    """Owns a ``lsim_ctx`` with the LUT tables, and runs the forward / backward chain."""
+    """Owns a ``lsim_ctx`` with the LUT tables, and runs the forward / backward chain.  With
+    ``response=None`` the context has no LUT: only the readout (``fee``) and the parametrized
+    current model (``larndsim_amd.param``) run on it."""
 
     def __init__(self, params, response, device="cuda"):
         if not torch.cuda.is_available():
@@ -161,6 +163,10 @@ class Simulator:
             cfg.long_diff_template[i] = float(s)
         self._ctx = C.c_void_p()
         _lib.check(self.lib.lsim_ctx_create(C.byref(self._ctx), C.byref(geo), C.byref(cfg)))
+        self.tape_bytes = int(self.lib.lsim_tape_bytes())
+        if response is None:  # no LUT: the readout and the parametrized current model only
+            self.lut = None
+            return
         lut = torch.as_tensor(np.asarray(response) if not isinstance(response, torch.Tensor) else response)
         lut = lut.to(device=self.device, dtype=torch.float32).contiguous()
         if lut.ndim != 3:
@@ -168,7 +174,6 @@ class Simulator:
         self.lut = lut
         _lib.check(self.lib.lsim_lut_set(self._ctx, C.c_void_p(lut.data_ptr()), lut.shape[0], lut.shape[1], lut.shape[2],
                                          self._stream()))
-        self.tape_bytes = int(self.lib.lsim_tape_bytes())
 
     def __del__(self):
         try:
@@ -497,6 +502,11 @@ def set_row_cost(pairs: int):
     _lib.check(_lib.load().lsim_set_row_cost(int(pairs)))
 
 
+def set_row_tile(cells: int):
+    """Side of the square pixel-cell tiles the waveform kernel processes rows in (1: row order)."""
+    _lib.check(_lib.load().lsim_set_row_tile(int(cells)))
+
+
 def _kernel_ms(runner: Runner, reps: int) -> float:
     runner.forward()
     torch.cuda.synchronize()
@@ -521,18 +531,28 @@ def autotune_row_waves(runner: Runner, candidates=(1, 2, 4, 8), reps=3):
     return best, res
 
 
-def autotune_rows(runner: Runner, waves=(1, 2, 4, 8), costs=(64, 256, 1024), reps=3):
-    """Tune the waveform kernel on the runner's workload: waves per row workgroup first (at the
-    default row cost), then the per-row cost the XCD bands are balanced with.  Both only change
-    the schedule and the per-wave partial sums; the settings stay in effect.  Returns
-    (waves, row cost, {setting: kernel ms})."""
+def autotune_rows(runner: Runner, waves=(1, 2, 4, 8), costs=(64, 256, 1024), tiles=(1, 8, 16, 32), reps=3):
+    """Tune the waveform kernel on the runner's workload, one knob at a time: waves per row
+    workgroup (at the default row cost and tile), then the per-row cost the XCD bands are balanced
+    with, then the side of the cell tiles the rows are processed in.  All three only change the
+    schedule and the per-wave partial sums; the settings stay in effect.  Returns (waves, row cost,
+    tile, {setting: kernel ms})."""
     set_row_cost(256)
+    set_row_tile(16)
     w, res_w = autotune_row_waves(runner, waves, reps)
-    best, best_ms, res_c = None, None, {}
-    for c in costs:
-        set_row_cost(c)
-        res_c[c] = _kernel_ms(runner, reps)
-        if best_ms is None or res_c[c] < best_ms:
-            best, best_ms = c, res_c[c]
-    set_row_cost(best)
-    return w, best, {**{f"waves={k}": v for k, v in res_w.items()}, **{f"row_cost={k}": v for k, v in res_c.items()}}
+
+    def pick(setter, values):
+        best, best_ms, res = None, None, {}
+        for v in values:
+            setter(v)
+            res[v] = _kernel_ms(runner, reps)
+            if best_ms is None or res[v] < best_ms:
+                best, best_ms = v, res[v]
+        setter(best)
+        return best, res
+
+    c, res_c = pick(set_row_cost, costs)
+    t, res_t = pick(set_row_tile, tiles)
+    return w, c, t, {**{f"waves={k}": v for k, v in res_w.items()}, **{f"row_cost={k}": v for k, v in res_c.items()},
+                     **{f"row_tile={k}": v for k, v in res_t.items()}}
+

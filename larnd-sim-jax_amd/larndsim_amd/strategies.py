@@ -102,6 +102,20 @@ class HipLUTProbabilisticSimulation(SimulationStrategy):
                     wfs=wfs)
 
 
+class HipParametrizedSimulation(SimulationStrategy):
+    """The parametrized current model on the GPU (strategies.py:136-149): the compacted hit arrays
+    of ``param.simulate_parametrized`` under the reference's keys.  ``rngkey`` seeds the normal
+    draws of mc_diff.  Forward only."""
+
+    def predict(self, params, tracks, fields, rngkey=0):
+        from .param import simulate_parametrized
+
+        adcs, x, y, z, ticks, hit_prob, event, hit_pixels = simulate_parametrized(
+            params, tracks, fields, rngseed=0 if rngkey is None else int(rngkey))
+        return dict(adcs=adcs, pixel_x=x, pixel_y=y, pixel_z=z, ticks=ticks, hit_prob=hit_prob, event=event,
+                    unique_pixels=hit_pixels)
+
+
 def _last_others(t):
     return t.grad_fn.others if t.grad_fn is not None else {}
 

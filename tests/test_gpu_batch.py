@@ -63,6 +63,25 @@ def test_run_twice_bitwise(sim, params4, batch):
     assert torch.equal(a.out["hit_adc"][:ha], b.out["hit_adc"][:hb])
 
 
+@pytest.mark.parametrize("tile,cost", [(1, 64), (8, 1024), (32, 256)])
+def test_schedule_settings_keep_outputs_bitwise(sim, params4, batch, tile, cost):
+    """The row order (cell tiles) and the work estimate of the XCD bands change only which
+    workgroup computes a row and when: every output is bitwise the default schedule's."""
+    from larndsim_amd import sim as S
+
+    tr, f, sizes = batch
+    a = sim.forward(params4, tr, f, n_events=len(sizes))
+    try:
+        S.set_row_tile(tile)
+        S.set_row_cost(cost)
+        b = sim.forward(params4, tr, f, n_events=len(sizes))
+    finally:
+        S.set_row_tile(16)
+        S.set_row_cost(256)
+    for key in ("pixel_ids", "wfs", "adc", "ticks"):
+        assert np.array_equal(_np(a, key), _np(b, key)), key
+
+
 def test_linearity_in_charge_bitwise(sim, params4, batch):
     """Doubling dE doubles n_e exactly, and every waveform value with it."""
     tr, f, sizes = batch

@@ -253,6 +253,16 @@ def test_new_entry_points_check_arguments(lib):
     assert lib.lsim_prob_fee(None, 1, 2000, 20, 10, 18, C.c_float(7e3), C.c_float(0.0), C.c_float(0.1),
                              C.c_float(1e-9), None, 1 << 30, None, None, None) < 0
     assert b"noise_sigma" in lib.lsim_last_error()
+    # the parametrized current model: modes, bin edges, sample count, NULLs (no device access)
+    pp = C.c_void_p(8)
+    assert lib.lsim_param_electrons(None, None, None, 0, 1, None, None, 6, None, None, None, None, None, None, None,
+                                    None) < 0
+    assert lib.lsim_param_rows(None, None, None, None, None, None, None, None, 0, None, None, 1, None, None) < 0
+    assert lib.lsim_param_rows(pp, None, None, None, None, None, None, None, 0, None, None, 1, None, None) < 0
+    # scheduling knobs: out-of-range values are refused on the host
+    assert lib.lsim_set_row_tile(0) < 0 and lib.lsim_set_row_tile(4097) < 0 and b"tile" in lib.lsim_last_error()
+    assert lib.lsim_set_row_tile(16) == 0
+    assert lib.lsim_set_row_cost(0) < 0 and lib.lsim_set_row_cost(256) == 0
     # the readout's adjoint: same limits, a workspace of the forward's size, no rows is a no-op
     bwd = lambda P, T, V, ws_bytes: lib.lsim_prob_fee_backward(  # noqa: E731
         P, T, V, 10, 18, C.c_float(7e3), C.c_float(900.0), C.c_float(0.1), None, ws_bytes, None, None, None, None, None)
