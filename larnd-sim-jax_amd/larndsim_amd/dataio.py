@@ -169,3 +169,166 @@ def group_events_into_batches(traj_event_ids, traj_lengths, max_batch_len, max_n
     members = keep[order]
     batches = [members[grp_start[a]:grp_start[b]].tolist() for a, b in zip(bounds[:-1], bounds[1:])]
     return batches, float(cum[bounds[-1] - 1])
+
+
+# ------------------------------------------------------------------------------------------
+# the on-disk segment files and the dataset of batches (optimize/dataio.py:44-45, 108-424)
+# ------------------------------------------------------------------------------------------
+
+def load_segments(filename, swap_xz=True):
+    """The ``segments`` compound dataset of a larnd-sim input file as a numpy structured array,
+    prepared as ``TracksDataset.__init__`` does (dataio.py:113-139): x and z columns swapped
+    (x, x_start, x_end <-> z, z_start, z_end), a zero ``t0`` column appended when absent, and
+    event_id / traj_id renamed to eventID / trackID.  Read with the package's own HDF5 reader."""
+    from numpy.lib import recfunctions as rfn
+
+    from .h5lite import read_dataset
+
+    tracks = read_dataset(filename, "segments")
+    if swap_xz:
+        for a, b in (("x_start", "z_start"), ("x_end", "z_end"), ("x", "z")):
+            ta, tb = np.copy(tracks[a]), np.copy(tracks[b])
+            tracks[a], tracks[b] = tb, ta
+    if "t0" not in tracks.dtype.names:
+        tracks = rfn.append_fields(tracks, "t0", np.zeros(tracks.shape[0]), usemask=False)
+    replace = {"event_id": "eventID", "traj_id": "trackID"}
+    tracks.dtype.names = tuple(replace.get(n, n) for n in tracks.dtype.names)
+    return tracks
+
+
+def np_from_structured(tracks) -> np.ndarray:
+    """(N, F) float32 of a structured array, columns in field order (dataio.py:44-45)."""
+    from numpy.lib import recfunctions as rfn
+
+    return np.asarray(rfn.structured_to_unstructured(tracks, copy=True, dtype=np.float32), dtype=np.float32)
+
+
+class TracksDataset:
+    """Batches of whole events from a segment file (dataio.py:108-424): the same selection, event
+    choice, event-grouped batch split and per-batch step count as the reference; a batch is
+    chopped, remapped and padded on the GPU (``chop_batch``).  ``__getitem__`` returns a float32
+    device tensor (N, F) in ``get_track_fields()`` order."""
+
+    def __init__(self, filename, nevents, max_nbatch=None, swap_xz=True, random_nevents=False, data_seed=42,
+                 track_len_sel=2., max_abs_costheta_sel=0.966, min_abs_segz_sel=15., track_z_bound=28.,
+                 max_batch_len=50, print_input=False, chopped=True, pad=True, electron_sampling_resolution=0.1,
+                 live_selection=False, device=None):
+        tracks = load_segments(filename, swap_xz)
+        self.track_fields = tracks.dtype.names
+        # load only what the batches can use, trimmed at an event boundary (dataio.py:145-157)
+        if max_batch_len is not None and max_nbatch is not None and max_nbatch > 0:
+            threshold = max_batch_len * (max_nbatch + 2)
+            cutoff = int(np.searchsorted(np.cumsum(tracks["dx"]), threshold, side="left"))
+            if cutoff < len(tracks):
+                other = tracks["eventID"][:cutoff + 1] != tracks["eventID"][cutoff]
+                if other.any():
+                    tracks = tracks[:int(np.where(other)[0][-1]) + 1]
+        # trajectories (dataio.py:162-224); the row lists index the loaded table, also with the
+        # live selection, whose keys come from the selected rows (as in the reference)
+        if live_selection:
+            sel = tracks[abs(tracks["z"]) < min_abs_segz_sel]
+            _, first = np.unique(sel[["eventID", "trackID"]], return_index=True)
+            first = np.sort(first)
+            last = np.r_[first[1:] - 1, len(sel) - 1]
+            s, e = sel[first], sel[last]
+            d = np.column_stack((s["x_start"] - e["x_end"], s["y_start"] - e["y_end"], s["z_start"] - e["z_end"]))
+            cos = np.abs(d @ np.array([0, 0, 1])) / (np.linalg.norm(d, axis=1) + 1e-10)
+            m = (np.sqrt((d ** 2).sum(axis=1)) > track_len_sel) & (cos < max_abs_costheta_sel)
+            m &= np.maximum(abs(s["z"]), abs(e["z"])) < track_z_bound
+            keys = np.ascontiguousarray(sel[np.repeat(m, last - first + 1)][["eventID", "trackID"]])
+        else:
+            keys = np.ascontiguousarray(tracks[["eventID", "trackID"]])
+        index, inverse = np.unique(keys, return_inverse=True)
+        self.traj_keys = np.ascontiguousarray(index)
+        self.tracks_struct = tracks
+        sorted_idx = np.argsort(inverse)
+        vals, starts = np.unique(inverse[sorted_idx], return_index=True)
+        ends = np.append(starts[1:], len(sorted_idx))
+        traj_rows = [np.empty(0, dtype=int) for _ in range(len(index))]
+        for v, a, b in zip(vals, starts, ends):
+            traj_rows[int(v)] = sorted_idx[a:b]
+        event_ids = np.array([tracks[traj_rows[i][0]]["eventID"] for i in range(len(traj_rows))])
+        uniq, first_idx = np.unique(event_ids, return_index=True)
+        ordered = uniq[np.argsort(first_idx)]
+        if nevents is not None and nevents > 0:
+            if random_nevents:
+                rng = np.random.default_rng(seed=data_seed)
+                chosen = rng.choice(ordered, size=nevents, replace=False) if nevents < len(ordered) else ordered
+            else:
+                chosen = ordered[:nevents]
+            traj_rows = [traj_rows[i] for i in np.where(np.isin(event_ids, chosen))[0]]
+        self.trajectory_row_indices = traj_rows
+        if max_batch_len is not None:
+            lengths = np.array([tracks[r]["dx"].sum() for r in traj_rows])
+            evs = np.array([tracks[r[0]]["eventID"] for r in traj_rows])
+            self.batch_traj_indices, self.tot_data_length = group_events_into_batches(evs, lengths, max_batch_len,
+                                                                                      max_nbatch)
+        else:
+            self.batch_traj_indices = [[i] for i in range(len(traj_rows))]
+            self.tot_data_length = float(np.sum([tracks[r]["dx"].sum() for r in traj_rows]))
+        if min(len(b) for b in self.batch_traj_indices) == 0:
+            raise ValueError("There exist some empty batch in the simulation input!")
+        self.batch_nsteps, self.batch_row_keys, self.batch_event_global_ids, self.batch_row_indices = [], [], [], []
+        for b in self.batch_traj_indices:
+            rows = np.concatenate([traj_rows[t] for t in b])
+            steps = np.maximum(np.ceil(tracks[rows]["dx"] / electron_sampling_resolution), 1).astype(int)
+            self.batch_nsteps.append(int(steps.sum()))
+            bk = np.ascontiguousarray(np.unique(tracks[rows][["eventID", "trackID"]]))
+            self.batch_row_keys.append(bk)
+            self.batch_event_global_ids.append(np.asarray(np.unique(bk["eventID"]), dtype=np.int64))
+            self.batch_row_indices.append(np.asarray(rows, dtype=np.int64))
+        self.max_batch_nsteps = int(max(self.batch_nsteps)) if self.batch_nsteps else 0
+        self.chopped, self.pad = chopped, pad
+        self.electron_sampling_resolution = electron_sampling_resolution
+        self.print_input = print_input
+        self.device = device
+
+    def __len__(self):
+        return len(self.batch_traj_indices)
+
+    def __getitem__(self, idx):
+        if idx < 0 or idx >= len(self):
+            raise IndexError("Batch index out of range")
+        rows = np.concatenate([self.trajectory_row_indices[t] for t in self.batch_traj_indices[idx]])
+        sel = self.tracks_struct[rows]
+        arr = np_from_structured(sel)
+        target = self.max_batch_nsteps if self.pad and self.max_batch_nsteps > 0 else None
+        if not self.chopped:  # remap and pad on the host (dataio.py:47-61, 357-373)
+            ids = self.batch_event_global_ids[idx]
+            src = np.asarray(sel["eventID"], dtype=np.int64)
+            loc = np.searchsorted(ids, src)
+            if np.any(loc >= ids.size) or not np.array_equal(ids[np.minimum(loc, ids.size - 1)], src):
+                raise ValueError("Failed to remap eventID to local indices")
+            f = list(self.track_fields)
+            ev = f.index("eventID")
+            arr[:, ev] = loc.astype(np.float32)
+            n = arr.shape[0]
+            if target is not None and target > n:
+                arr = np.concatenate([arr, np.zeros((target - n, arr.shape[1]), np.float32)])
+                bad = np.zeros(target, bool)
+                bad[n:] = True
+                bad |= (arr[:, ev] >= 0) & ~np.isin(arr[:, ev].astype(np.int64), np.arange(ids.size))
+                arr[bad, ev] = -1
+                for name in ZERO_PAD_FIELDS:
+                    if name in f:
+                        arr[bad, f.index(name)] = 0
+                for name in NEG_ONE_PAD_FIELDS:
+                    if name in f:
+                        arr[bad, f.index(name)] = -1
+            dev = self.device if self.device is not None else torch.device("cuda")
+            return torch.from_numpy(arr).to(dev)
+        return chop_batch(arr, self.track_fields, precision=self.electron_sampling_resolution, target_len=target,
+                          source_event_ids=sel["eventID"], batch_event_ids=self.batch_event_global_ids[idx],
+                          device=self.device)
+
+    def get_track_fields(self):
+        return self.track_fields
+
+    def get_batch_global_event_ids(self, idx=None):
+        return self.batch_event_global_ids if idx is None else self.batch_event_global_ids[idx]
+
+    def get_batch_row_keys(self):
+        return self.batch_row_keys
+
+    def get_batch_row_indices(self, idx=None):
+        return self.batch_row_indices if idx is None else self.batch_row_indices[idx]

@@ -1,8 +1,9 @@
 """The probabilistic readout on the GPU (lsim_prob_fee, float32) against the float64 restatement
 of the reference's beam search (oracle/probfee.py) on hand-built pulse trains and on simulated
-waveforms: expected charges to 2e-6 of their scale, hit probabilities to 5 % relative + 2e-4
-absolute, each hit's total probability to 1e-3 + 1 %.  Later hit indices are compared on rows
-whose top-V selections were not near ties."""
+waveforms: expected charges to 2e-6 of their scale; hit probabilities within twice the error of
+the same function evaluated in float32 (torch on the CPU, the oracle's path selections) plus 2e-5;
+each hit's total probability to 1e-4 + 0.1 %.  Later hit indices are compared on rows whose top-V
+selections were not near ties."""
 import numpy as np
 import pytest
 import torch
@@ -29,11 +30,16 @@ def _compare(params, w, stop=1e-9):
 
     lp, q = get_adc_values_average_noise_vmap(params, torch.from_numpy(w).cuda(), stop)
     lp, q = lp.cpu().numpy().astype(np.float64), q.cpu().numpy().astype(np.float64)
-    rl, rq, sels = PF.prob_fee(w, params.t_sampling, int(params.fee_paths_scaling), int(params.MAX_ADC_VALUES),
-                               params.adc_interval, params.DISCRIMINATION_THRESHOLD, params.RESET_NOISE_CHARGE, stop)
+    args = (params.t_sampling, int(params.fee_paths_scaling), int(params.MAX_ADC_VALUES), params.adc_interval,
+            params.DISCRIMINATION_THRESHOLD, params.RESET_NOISE_CHARGE)
+    rl, rq, sels, tops, n_steps = PF.prob_fee_tape(w, *args, stop)
+    # the same function in float32 (torch on the CPU, the oracle's selections): the rounding scale
+    l32, _ = PF.prob_fee_torch(torch.from_numpy(w), *args, tops, n_steps)
+    p32 = np.exp(l32.double().numpy())
     V = int(params.fee_paths_scaling)
     scale = np.abs(rq).max() + 1.0
     compared = 0
+    worst_g, worst_32 = 0.0, 0.0
     for r in range(w.shape[0]):
         for s in range(lp.shape[1]):
             assert np.allclose(q[r, s], rq[r, s], rtol=0, atol=2e-6 * scale), (r, s)
@@ -41,12 +47,15 @@ def _compare(params, w, stop=1e-9):
             # (sigmoid of 1000 (a - b)) turns float32 rounding of the z differences of nearly flat
             # ticks into log-probability changes of ~1e-2 where the probability is small
             pg, pr = np.exp(lp[r, s]), np.exp(rl[r, s])
-            assert np.allclose(pg, pr, rtol=5e-2, atol=2e-4), (r, s, np.abs(pg - pr).max())
-            assert abs(pg.sum() - pr.sum()) < 1e-3 + 1e-2 * pr.sum(), (r, s)
+            eg, e32 = np.abs(pg - pr).max(), np.abs(p32[r, s] - pr).max()
+            worst_g, worst_32 = max(worst_g, eg), max(worst_32, e32)
+            assert eg <= 2 * e32 + 2e-5, (r, s, eg, e32)
+            assert abs(pg.sum() - pr.sum()) < 1e-4 + 1e-3 * pr.sum(), (r, s)
             compared += 1
             srt = np.sort(sels[r, s])[::-1]
             if np.isfinite(srt[V]) and srt[V - 1] > -20.0 and srt[V - 1] - srt[V] < 1e-3 * max(1.0, abs(srt[V - 1])):
                 break  # near tie among paths that carry probability: later steps may follow others
+    print(f"[probfee] worst probability error: GPU {worst_g:.2e}, torch float32 {worst_32:.2e}", flush=True)
     return compared
 
 
