@@ -591,6 +591,8 @@ def main():
                     help="profiling: only the probabilistic readout on the 200k-segment event's rows")
     ap.add_argument("--batched-bwd-only", action="store_true",
                     help="profiling: only the 32-event batched forward+backward graph replays (no baseline)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="profiling: only the 128-event headline (no other leg, parity check or CPU baseline)")
     ap.add_argument("--profile-batch", action="store_true",
                     help="profiling: only eager forwards of the 128-event batch (for counter passes)")
     ap.add_argument("--record-traffic", metavar="DIR",
@@ -607,6 +609,9 @@ def main():
     if a.record_counters:
         print(json.dumps(record_counters(a.record_counters)["derived"]))
         return
+    if a.headline_only:
+        a.no_bwd = a.no_batched_bwd = a.no_prob_fee = a.no_sdtw = a.no_chop = a.no_cpu = True
+        a.fit_steps = a.fit_wfs_steps = 0
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events would shard across "
               "GPUs with no exchange in the forward and one 8-value all-reduce per gradient step)", file=sys.stderr)
@@ -653,6 +658,8 @@ def main():
         brun = S.Runner(simr, params, torch.from_numpy(bt).to(dev), bf, n_events=128, want_wfs=True)
         if a.waves:
             S.set_row_waves(a.waves)
+            S.set_row_cost(a.row_cost)
+            S.set_row_tile(a.row_tile)
         for i in range(a.steps):
             brun.forward()
             torch.cuda.synchronize()
@@ -711,6 +718,12 @@ def main():
     del brun, btr, g
 
     extra = {}
+    if a.headline_only:
+        print(json.dumps({"metric": "segments/s", "value": seg_s, "ms_per_step": ms_step, "kernel_ms": k_avg,
+                          "achieved_gbs": achieved, "config": {"segments": NB, "pixel_rows": PB, "row_waves": waves,
+                                                                "row_cost": row_cost, "row_tile": row_tile}}),
+              flush=True)
+        return
     # ---- the 200k-segment event alone ----
     log("[bench] generating the 200k-segment event")
     tracks, fields = synthetic.workload("c2")
