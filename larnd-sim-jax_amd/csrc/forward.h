@@ -777,8 +777,8 @@ constexpr int kMaxCells = 17 * 17;  // (2n+1)^2 for number_pix_neighbors <= 8
 // order, is the segment cell of the pixel (px - a, py - b); it is listed when cls(a, b) >= 0 and
 // it holds segments, ordered by (cls, i); cnt(begin, end) gives how many of the cell's leading
 // segments to list (a cell with none is left out).  One thread per cell loads the cell's segment range;
-// each listed cell then finds its place and the number of segments before it by one pass over
-// the keys in LDS.  Writes c_beg, c_ab (a + 64 << 8 | b + 64), c_cum[0..k] (exclusive sums of the
+// then wave 0 ranks the listed cells, 64 at a time per class, with a ballot (the place) and a wave
+// scan of the counts (the segments before each).  Writes c_beg, c_ab (a + 64 << 8 | b + 64), c_cum[0..k] (exclusive sums of the
 // cells' segment counts) and ncls[c] = number of listed cells of class <= c, c < 3.  scratch
 // holds 3 * kMaxCells ints; the call begins and ends with every thread of the block in it.
 template <int NT, class Cls, class Cnt>
@@ -792,10 +792,6 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
   int* s_key = scratch;
   int* s_cnt = scratch + kMaxCells;
   int* s_beg = scratch + 2 * kMaxCells;
-  if (t == 0) {
-    c_cum[0] = 0;
-    ncls[0] = ncls[1] = ncls[2] = 0;
-  }
   for (int i = t; i < ncell; i += NT) {
     const int a = i / side - d.ne, b = i % side - d.ne;
     const int c = cls(a, b);
@@ -815,42 +811,44 @@ __device__ __forceinline__ void list_cells(const Dev& d, const int2* __restrict_
     s_beg[i] = beg;
   }
   __syncthreads();
-  // each listed cell: its rank and the segments before it; the last cell of class 0, of classes
-  // <= 1 and of all classes writes that count (the totals stay 0 when there is no such cell)
-  for (int i = t; i < ncell; i += NT) {
-    const int key = s_key[i];
-    if (key < 0) continue;
-    const int c = key / ncell;
-    int rank = 0, before = 0, after0 = 0, after1 = 0, after = 0;
-    for (int j = 0; j < ncell; ++j) {
-      const int kj = s_key[j];
-      if (kj < 0) continue;
-      if (kj < key) {
-        ++rank;
-        before += s_cnt[j];
-      } else if (kj > key) {
-        const int cj = kj / ncell;
-        after0 += cj == 0 ? 1 : 0;
-        after1 += cj <= 1 ? 1 : 0;
-        ++after;
+  // ranks in (class, i) order: wave 0 walks the cells class by class, 64 at a time; a ballot gives
+  // each listed cell its rank and a wave scan of the counts the segments before it
+  if (t < kWave) {
+    const int lane = t;
+    int rank0 = 0, before0 = 0;
+    for (int c = 0; c < 3; ++c) {
+      for (int i0 = 0; i0 < ncell; i0 += kWave) {
+        const int i = i0 + lane;
+        const int key = i < ncell ? s_key[i] : -1;
+        const bool on = key >= 0 && key / ncell == c;
+        const uint64_t m = __ballot(on);
+        if (m == 0) continue;
+        const int n = on ? s_cnt[i] : 0;
+        int inc = n;
+#pragma unroll
+        for (int dd = 1; dd < kWave; dd <<= 1) {
+          const int u = __shfl_up(inc, dd, kWave);
+          if (lane >= dd) inc += u;
+        }
+        if (on) {
+          const int rk = rank0 + __popcll(m & ((1ull << lane) - 1));
+          const int a = i / side - d.ne, b = i % side - d.ne;
+          c_beg[rk] = s_beg[i];
+          c_cum[rk] = before0 + (inc - n);
+          c_ab[rk] = ((a + 64) << 8) | (b + 64);
+        }
+        rank0 += __popcll(m);
+        before0 += __shfl(inc, kWave - 1, kWave);
       }
+      if (lane == 0) ncls[c] = rank0;
     }
-    const int a = i / side - d.ne, b = i % side - d.ne;
-    c_beg[rank] = s_beg[i];
-    c_cum[rank] = before;
-    c_ab[rank] = ((a + 64) << 8) | (b + 64);
-    if (c == 0 && after0 == 0) ncls[0] = rank + 1;
-    if (c <= 1 && after1 == 0) ncls[1] = rank + 1;
-    if (after == 0) {
-      ncls[2] = rank + 1;
-      c_cum[rank + 1] = before + s_cnt[i];
-    }
+    if (lane == 0) c_cum[rank0] = before0;
   }
   __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------------
-// work-balanced XCD bands of pixel rows for the persistent row kernels
+// work-balanced XCD bands of pixel rows for the row kernels
 // ------------------------------------------------------------------------------------------
 //
 // Workgroup i of a launch runs on XCD i % kXcd, so each XCD is given one contiguous band of rows
@@ -1026,7 +1024,12 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         return !near ? 0 : ((a == 0 && b == 0) ? 2 : 1);
       },
       [](int b, int e) { return e - b; }, reinterpret_cast<int*>(acc), c_beg, c_cum, c_ab, ncls);
-  for (int o = t; o < WAVES * rstride; o += NT) acc[o] = 0.f;
+  {
+    // 16-byte stores (smem is 16-byte aligned), then the remainder
+    const int nz = WAVES * rstride, n4 = nz / 4;
+    for (int o = t; o < n4; o += NT) reinterpret_cast<float4*>(acc)[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int o = 4 * n4 + t; o < nz; o += NT) acc[o] = 0.f;
+  }
   __syncthreads();
 
   float* accw = acc + wid * rstride;
