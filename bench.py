@@ -265,7 +265,7 @@ def record_traffic(d, kernel_ms):
 
 
 COUNTERS_JSON = os.path.join(ROOT, "profiles", "kernel_counters_128_events.json")
-COUNTER_KERNELS = ("k_rows_fwd", "k_center", "k_records")
+COUNTER_KERNELS = ("k_rows_fwd", "k_records")
 
 
 def _counter_means(d):

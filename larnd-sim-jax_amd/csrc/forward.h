@@ -710,17 +710,12 @@ __device__ __forceinline__ void side_bins(const Dev& d, const Tables& tb, const 
   }
 }
 
-// Segment-parallel window of the diffusion bins that land on the segment's own (main) pixel:
-// vc[s][j] = sum_bins n_bin * (b T_idx + a T_{idx-1} + c T_{idx+1})[cx][cy][j]
-// (sim_jax.py:186-194), stored padded like the window tables.  One wave per segment (sorted
-// order); the waveform kernel then adds these like a neighbour stripe.
+// Window of the diffusion bins of segment R that land on its own (main) pixel, charge included:
+// V[j] = sum_bins n_bin * (b T_idx + a T_{idx-1} + c T_{idx+1})[cx][cy][j] (sim_jax.py:186-194), at
+// the padded positions lane + 63 c (the chunks of store_window), each distinct LUT cell once.
 template <int NB>
-__global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* __restrict__ rec,
-                                                const uint32_t* __restrict__ keys, int64_t n, float* __restrict__ vc) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
-  if (s >= n || keys[s] >= (uint32_t)d.n_cells) return;
-  const SegRec R = rec[s];
+__device__ __forceinline__ void center_bins(const Dev& d, const Tables& tb, const SegRec& R, int lane, float& v0,
+                                            float& v1, float& v2) {
   const int nsb = d.nsb, cm = d.cm;
   const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
   bool okx[NB], oky[NB];
@@ -729,10 +724,8 @@ __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* 
   axis_merge<NB>(ux, 0, nsb, R.wx, R.ne, okx, cxs, WX);
   axis_merge<NB>(uy, 0, nsb, R.wy, 1.f, oky, cys, WY);
   const float la = R.la, lb = R.lb, lc = R.lc;
-  const int idx = R.idx;
-  const float3* T3 = tb.TM3 + (int64_t)idx * cm * cm * kWinPad + lane;
-  // padded positions p = lane, lane + 64, lane + 128 (kWinPad = 3 * 64)
-  float v1 = 0.f, v2 = 0.f, v3 = 0.f;
+  const float3* T3 = tb.TM3 + (int64_t)R.idx * cm * cm * kWinPad + lane;
+  v0 = v1 = v2 = 0.f;
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     if (!okx[i]) continue;
@@ -741,27 +734,12 @@ __global__ __launch_bounds__(256) void k_center(Dev d, Tables tb, const SegRec* 
       if (!oky[jj]) continue;
       const float c = WX[i] * WY[jj];
       const float3* tp = T3 + ((int64_t)cxs[i] * cm + cys[jj]) * kWinPad;
-      const float3 q1 = tp[0], q2 = tp[kWave], q3 = tp[2 * kWave];
+      const float3 q0 = tp[0], q1 = tp[kChunk], q2 = tp[2 * kChunk];
+      v0 += c * (q0.y * lb + q0.x * la + q0.z * lc);
       v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
       v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
-      v3 += c * (q3.y * lb + q3.x * la + q3.z * lc);
     }
   }
-  float* o = vc + (int64_t)s * kWinPad + lane;
-  o[0] = v1;
-  o[kWave] = v2;
-  o[2 * kWave] = v3;
-}
-
-// Center pair from the precomputed window: V = vc[s] (charge included), Delta = dtc[s].
-__device__ __forceinline__ void center_load(const Dev& d, const float* __restrict__ vc, const float* __restrict__ dtc,
-                                            int s, int lane, FarLoads& q) {
-  const float* V = vc + (int64_t)s * kWinPad + lane;
-  q.t0 = V[0];
-  q.t1 = V[kChunk];
-  q.t2 = V[2 * kChunk];
-  (void)d;
-  q.D = dtc[s];
 }
 
 // Dynamic LDS bytes of the waveform kernel: the per-wave partial rows, reused afterwards for the
@@ -986,7 +964,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
                                                             const int2* __restrict__ rng, const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
                                                             const int* __restrict__ band, const int* __restrict__ order,
-                                                            const float* __restrict__ vc, const float* __restrict__ dtot,
+                                                            const float* __restrict__ dtot,
                                                             const PairRec* __restrict__ prs, int64_t nseg, RowOut out) {
   constexpr int NT = WAVES * kWave;
   extern __shared__ __align__(16) char smem[];
@@ -1133,30 +1111,18 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
       }
     }
   }
-  // the centre cell: precomputed windows of the bins on this pixel, grouped like the far pairs
+  // the centre cell: the window of each segment's bins on this pixel, from the interpolated
+  // templates of its distinct LUT cells, one segment per wave at a time
   const float* dtc = dtot + (int64_t)(d.ne * (2 * d.ne + 1) + d.ne) * nseg;
   for (int c = nsc; c < nct; ++c) {
     const int beg = __builtin_amdgcn_readfirstlane(c_beg[c]);
     const int cnt = __builtin_amdgcn_readfirstlane(c_cum[c + 1] - c_cum[c]);
-    for (int k = wid; k < cnt; k += WAVES * kFarGroup) {
-      PairRec P[kFarGroup];
-      int sg[kFarGroup];
-      float sc[kFarGroup];
-#pragma unroll
-      for (int g = 0; g < kFarGroup; ++g) {
-        const int j = k + g * WAVES;
-        sg[g] = beg + (j < cnt ? j : k);
-        P[g] = prs[sg[g]];
-        sc[g] = j < cnt ? 1.f : 0.f;  // padding slot of the last group: adds exactly zero
-      }
-      FarLoads q[kFarGroup];
-#pragma unroll
-      for (int g = 0; g < kFarGroup; ++g) center_load(d, vc, dtc, sg[g], lane, q[g]);
-      // unconditional stores (see the far pairs)
-#pragma unroll
-      for (int g = 0; g < kFarGroup; ++g)
-        store_window(accw, lane, q[g].t0, q[g].t1, q[g].t2, sc[g], 1.f - P[g].f, P[g].f, sc[g] * q[g].D, P[g].S, d.L,
-                     d.nout, (P[g].flags & 1) != 0);
+    for (int k = wid; k < cnt; k += WAVES) {
+      const int sg = beg + k;
+      float v0, v1, v2;
+      center_bins<NB>(d, tb, rec[sg], lane, v0, v1, v2);
+      const PairRec P = prs[sg];
+      store_window(accw, lane, v0, v1, v2, 1.f, 1.f - P.f, P.f, dtc[sg], P.S, d.L, d.nout, (P.flags & 1) != 0);
     }
   }
   __syncthreads();

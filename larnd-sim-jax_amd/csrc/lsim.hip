@@ -185,7 +185,6 @@ struct Ws {
   int* row_work;  // [n] estimated work of each pixel row in processing order (XCD band cuts)
   int* band;      // [kXcd + 1] first processing slot of each XCD's band
   uint32_t *tk0, *tv0, *tk1, *tv1;  // [n + 1] tile sort ping-pong; the processing order lands in tv0 or tv1
-  float* vc;    // [n][kWinPad] centre-pixel windows of each segment (sorted order, padded)
   float* dtot;  // [(2 nn + 1)^2][n] boundary corrections of every (segment, row offset) pair
   size_t bytes;
 };
@@ -216,18 +215,15 @@ Ws layout(void* base, int64_t n, int64_t n_cells) {
   w.tv0 = c.take<uint32_t>(n + 1);
   w.tk1 = c.take<uint32_t>(n + 1);
   w.tv1 = c.take<uint32_t>(n + 1);
-  w.vc = nullptr;
   w.dtot = nullptr;
   w.bytes = c.off + 256;
   return w;
 }
 
-// The centre windows (kWinPad floats per segment) and the pair corrections ((2 nn + 1)^2 floats
-// per segment); carved after the rest.
-void layout_center(Ws& w, void* base, int64_t n, int nn) {
+// The pair corrections ((2 nn + 1)^2 floats per segment), carved after the rest.
+void layout_corrections(Ws& w, void* base, int64_t n, int nn) {
   Carve c(base);
   c.off = w.bytes;
-  w.vc = c.take<float>(n * kWinPad);
   w.dtot = c.take<float>((int64_t)(2 * nn + 1) * (2 * nn + 1) * n);
   w.bytes = c.off + 256;
 }
@@ -293,7 +289,7 @@ void launch_rows_nb(const Dev& d, const Tables& tb, const Ws& w, int64_t n_seg, 
   const size_t lds = rows_lds_bytes(WAVES, d.nout);
   const int64_t grid = kXcd * band_slots(max_rows);
   hipLaunchKernelGGL((k_rows_fwd<NB, WAVES>), dim3((unsigned)grid), dim3(WAVES * kWave), lds, st, d, tb, w.rec, w.rng,
-                     w.row_cell, w.n_rows, max_rows, w.band, row_order(w, d), w.vc, w.dtot, w.prs, n_seg, ro);
+                     w.row_cell, w.n_rows, max_rows, w.band, row_order(w, d), w.dtot, w.prs, n_seg, ro);
 }
 
 template <int WAVES>
@@ -480,7 +476,7 @@ int lsim_forward_query(lsim_ctx* c, int64_t n_seg, int32_t n_events, size_t* ws_
   int rc = make_dev(c, nullptr, n_events, d);
   if (rc) return rc;
   Ws w = layout(nullptr, n_seg, d.n_cells);
-  layout_center(w, nullptr, n_seg, d.ne);
+  layout_corrections(w, nullptr, n_seg, d.ne);
   *ws_bytes = w.bytes;
   if (max_rows_bound) {
     const int64_t plane_pixels = (int64_t)n_events * d.ntpc * d.nx * d.ny;
@@ -501,7 +497,7 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
   if (rc) return rc;
   const int64_t n = seg->n;
   Ws w = layout(ws, n, d.n_cells);
-  layout_center(w, ws, n, d.ne);
+  layout_corrections(w, ws, n, d.ne);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small: need " + std::to_string(w.bytes) + " bytes");
   SegIn in = make_segin(seg);
   int* status = status_dev;
@@ -551,18 +547,6 @@ static int forward_rows_impl(lsim_ctx* c, const Dev& d, const Ws& w, int64_t n_s
   ro.hit_cnt = w.hit_cnt;
   ro.pixel_ids = out->pixel_ids;
   hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, w.n_rows, out->n_rows, max_rows, out->status);
-  if (n_seg > 0) {
-    uint32_t *keys, *vals;
-    sorted_buffers(w, d, keys, vals);
-    const unsigned g = (unsigned)((n_seg + 3) / 4);
-    if (d.nb == 5) {
-      hipLaunchKernelGGL(k_center<5>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-    } else if (d.nb == 3) {
-      hipLaunchKernelGGL(k_center<3>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-    } else {
-      hipLaunchKernelGGL(k_center<1>, dim3(g), dim3(256), 0, st, d, tb, w.rec, keys, n_seg, w.vc);
-    }
-  }
   if (max_rows > 0) {
     const int rc = launch_row_bands(d, w, max_rows, st);
     if (rc) return rc;
@@ -612,7 +596,7 @@ int lsim_forward_rows(lsim_ctx* c, const lsim_params* p, int64_t n_seg, int32_t 
   rc = make_dev(c, p, n_events, d);
   if (rc) return rc;
   Ws w = layout(ws, n_seg, d.n_cells);
-  layout_center(w, ws, n_seg, d.ne);
+  layout_corrections(w, ws, n_seg, d.ne);
   if (w.bytes > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
   // rows never exceed segments; the per-row workspace buffers hold max(n, 1) entries
   if (max_rows > (n_seg > 1 ? n_seg : 1)) max_rows = n_seg > 1 ? n_seg : 1;
