@@ -662,67 +662,21 @@ __device__ __forceinline__ bool bins_reach(int u, int a, int nsb) {
   return a == 0 ? true : (a > 0 ? u + (NB - 1 - SYM) >= nsb : u - SYM < 0);
 }
 
-// Window of the diffusion bins (interpolated templates) of segment R that land on a side row,
-// pixel offset (a, b) != (0, 0) with |a|, |b| <= 1, charge included.  TM3[idx][cx][cy][j] holds
-// (T_{idx-1}, T_idx, T_{idx+1}) so one 12-byte load serves the three Lagrange nodes.  Bin i of the
-// segment lies floor((u + i - SYM) / nsb) in {-1, 0, 1} pixels from the main pixel.
+// Window of the diffusion bins of segment R that land on the pixel (a, b) from its main pixel
+// (|a|, |b| <= 1), charge included: V[j] = sum_bins n_bin * (lb T_idx + la T_{idx-1} +
+// lc T_{idx+1})[cx][cy][j] (sim_jax.py:186-194), at the padded positions lane + 63 k (the chunks of
+// store_window), each distinct LUT cell once.  Bin i of the segment lies floor((u + i - SYM) / nsb)
+// in {-1, 0, 1} pixels from the main pixel.
 template <int NB>
-__device__ __forceinline__ void side_bins(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, int lane,
-                                          float& v0, float& v1, float& v2) {
-  constexpr int SYM = (NB - 1) / 2;
-  const int nsb = d.nsb, cm = d.cm;
-  const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
-  int cxs[NB], cys[NB];
-  float wxs[NB], wys[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) {
-    int vx = ux + i - SYM, vy = uy + i - SYM;
-    const int dx = vx < 0 ? -1 : (vx >= nsb ? 1 : 0);
-    const int dy = vy < 0 ? -1 : (vy >= nsb ? 1 : 0);
-    vx -= dx * nsb;
-    vy -= dy * nsb;
-    cxs[i] = offset_bin(vx, 0, nsb);
-    cys[i] = offset_bin(vy, 0, nsb);
-    wxs[i] = (dx == a) ? R.ne * R.wx[i] : 0.f;  // zero weight: the bin lands on another pixel
-    wys[i] = (dy == b) ? R.wy[i] : 0.f;
-  }
-  const float la = R.la, lb = R.lb, lc = R.lc;
-  v0 = v1 = v2 = 0.f;
-  const float3* T3 = tb.TM3 + (int64_t)R.idx * cm * cm * kWinPad + lane;
-#pragma unroll
-  for (int i = 0; i < NB; ++i) {
-    // one uniform branch per row of bins: the row's 3 * NB loads overlap, rows do not (bounds
-    // the live registers); bins of the row that land elsewhere load but add zero
-    if (wxs[i] != 0.f) {
-#pragma unroll
-      for (int jj = 0; jj < NB; ++jj) {
-        const float c = wxs[i] * wys[jj];
-        const int64_t coff = (int64_t)cxs[i] * cm + cys[jj];
-        const float3* tp = T3 + coff * kWinPad;
-        const float3 q0 = tp[0];
-        const float3 q1 = tp[kChunk];
-        const float3 q2 = tp[2 * kChunk];
-        v0 += c * (q0.y * lb + q0.x * la + q0.z * lc);
-        v1 += c * (q1.y * lb + q1.x * la + q1.z * lc);
-        v2 += c * (q2.y * lb + q2.x * la + q2.z * lc);
-      }
-    }
-  }
-}
-
-// Window of the diffusion bins of segment R that land on its own (main) pixel, charge included:
-// V[j] = sum_bins n_bin * (b T_idx + a T_{idx-1} + c T_{idx+1})[cx][cy][j] (sim_jax.py:186-194), at
-// the padded positions lane + 63 c (the chunks of store_window), each distinct LUT cell once.
-template <int NB>
-__device__ __forceinline__ void center_bins(const Dev& d, const Tables& tb, const SegRec& R, int lane, float& v0,
-                                            float& v1, float& v2) {
+__device__ __forceinline__ void pixel_bins(const Dev& d, const Tables& tb, const SegRec& R, int a, int b, int lane,
+                                           float& v0, float& v1, float& v2) {
   const int nsb = d.nsb, cm = d.cm;
   const int ux = R.uxy & 0xffff, uy = R.uxy >> 16;
   bool okx[NB], oky[NB];
   int cxs[NB], cys[NB];
   float WX[NB], WY[NB];
-  axis_merge<NB>(ux, 0, nsb, R.wx, R.ne, okx, cxs, WX);
-  axis_merge<NB>(uy, 0, nsb, R.wy, 1.f, oky, cys, WY);
+  axis_merge<NB>(ux, a, nsb, R.wx, R.ne, okx, cxs, WX);
+  axis_merge<NB>(uy, b, nsb, R.wy, 1.f, oky, cys, WY);
   const float la = R.la, lb = R.lb, lc = R.lc;
   const float3* T3 = tb.TM3 + (int64_t)R.idx * cm * cm * kWinPad + lane;
   v0 = v1 = v2 = 0.f;
@@ -1105,7 +1059,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         const int ab = __builtin_amdgcn_readlane(rab, j);
         const SegRec R = rec[__builtin_amdgcn_readlane(rsg, j)];
         float v0, v1, v2;
-        side_bins<NB>(d, tb, R, (ab >> 8) - 64, (ab & 255) - 64, lane, v0, v1, v2);
+        pixel_bins<NB>(d, tb, R, (ab >> 8) - 64, (ab & 255) - 64, lane, v0, v1, v2);
         const int S = d.nt - d.L - R.k0;
         store_window(accw, lane, v0, v1, v2, 1.f, 1.f - R.f, R.f, 0.f, S, d.L, d.nout, S >= 2 && S + d.L <= d.nout);
       }
@@ -1120,7 +1074,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     for (int k = wid; k < cnt; k += WAVES) {
       const int sg = beg + k;
       float v0, v1, v2;
-      center_bins<NB>(d, tb, rec[sg], lane, v0, v1, v2);
+      pixel_bins<NB>(d, tb, rec[sg], 0, 0, lane, v0, v1, v2);
       const PairRec P = prs[sg];
       store_window(accw, lane, v0, v1, v2, 1.f, 1.f - P.f, P.f, dtc[sg], P.S, d.L, d.nout, (P.flags & 1) != 0);
     }
