@@ -318,30 +318,61 @@ __device__ __forceinline__ int wave_min_i(int v) {
   return v;
 }
 
-// Self-trigger of one row (fee_jax.py:168-279, noise off) on Q held in LDS, then digitisation
-// (fee_jax.py:55-71).  Q and Q0 hold the float64 cumulative charge; Q is consumed.  The passes'
-// decisions go to LDS; wave 0 then writes the outputs and the tape with one lane per pass (and
-// one lane per tape point for the clamp masks).
+// Self-trigger of one row (fee_jax.py:168-279, noise off), then digitisation (fee_jax.py:55-71).
+// The float64 cumulative charge Q stays in registers: thread t holds ticks t * TPER + u in q[u]
+// (as finish_row leaves them) and takes the tick after its last one from the next lane (DPP, or
+// LDS across waves once, then updated by the same resets).  Q0 (LDS, written only with a tape)
+// is the unreset copy.  The passes' decisions go to LDS from the threads that own the ticks;
+// wave 0 then writes the outputs and the tape with one lane per pass (and one lane per tape point
+// for the clamp masks).
 template <int NT>
-__device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int plane, const RowOut& out, int* sh_int) {
+__device__ void fee_row(const Dev& d, double (&q)[kMaxOut / NT], const double* Q0, int r, int plane,
+                        const RowOut& out, int* sh_int) {
   constexpr int TPER = kMaxOut / NT;
+  constexpr int NW = NT / kWave;
   __shared__ int s_pe[kMaxH], s_pg[kMaxH], s_pi[kMaxH];
   __shared__ double s_pa[kMaxH];
-  __shared__ double s_qmax[NT / kWave];
+  __shared__ double s_qmax[NW], s_first[NW];
+  __shared__ double s_qg, s_gnone;
   const int t = threadIdx.x;
   const int nout = d.nout;
   const int lane = t & (kWave - 1), wid = t / kWave;
   const int H = d.H;
-  // ticks interleaved over the threads (o = t + NT u): a wave's lanes read consecutive doubles
+  const int o0 = t * TPER;
+  // Q[o] into v when this thread holds tick o (register indices fixed at compile time)
+  auto held = [&](int o, double& v) {
+    bool own = false;
+#pragma unroll
+    for (int u = 0; u < TPER; ++u)
+      if (o0 + u == o) {
+        v = q[u];
+        own = true;
+      }
+    return own;
+  };
+  // reset index of a pass without a crossing (i* = nout - 2), and its window end
+  int g_none = nout - 2 + 1 + d.interval + 1;
+  if (g_none >= nout) g_none = nout - 1;
+  int e_none = nout - 2 + 1 + d.interval;
+  if (e_none >= nout) e_none = nout - 1;
+  if (lane == 0) s_first[wid] = q[0];
+  {
+    double v;
+    if (held(g_none, v)) s_gnone = v;
+  }
+  __syncthreads();
+  double nx = __shfl_down(q[0], 1, kWave);  // Q[o0 + TPER]
+  if (lane == kWave - 1) nx = wid + 1 < NW ? s_first[wid + 1] : 0.0;
+  const double q_gnone = s_gnone;
   for (int k = 0; k < H; ++k) {
     int cand = 0x7fffffff;
     double qmax = -1.0e300;  // max of Q[1 .. nout-1], the values a crossing needs at or above thr
 #pragma unroll
     for (int u = 0; u < TPER; ++u) {
-      const int o = t + NT * u;
+      const int o = o0 + u;
       if (o <= nout - 2) {
-        const double q1 = Q[o + 1];
-        if (q1 >= d.thr && Q[o] <= d.thr && o < cand) cand = o;
+        const double q1 = u + 1 < TPER ? q[u + 1] : nx;
+        if (q1 >= d.thr && q[u] <= d.thr && o < cand) cand = o;
         qmax = q1 > qmax ? q1 : qmax;
       }
     }
@@ -358,30 +389,24 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
     __syncthreads();
     int istar = 0x7fffffff;
     double bmax = -1.0e300;
-    for (int w = 0; w < NT / kWave; ++w) {
+    for (int w = 0; w < NW; ++w) {
       istar = sh_int[w] < istar ? sh_int[w] : istar;
       bmax = s_qmax[w] > bmax ? s_qmax[w] : bmax;
     }
-    // reset index of a pass without a crossing (i* = nout - 2)
-    int g_none = nout - 2 + 1 + d.interval + 1;
-    if (g_none >= nout) g_none = nout - 1;
-    if (bmax < d.thr && (k > 0 || Q[g_none] >= 0.0)) {
+    if (bmax < d.thr && (k > 0 || q_gnone >= 0.0)) {
       // Nothing reaches the threshold: no crossing in this pass, and none later either.  A reset
       // Q <- max(Q - Q[g], 0) with Q[g] >= 0 never raises a value above max(Q, 0) < thr.  After
       // the first pass Q >= 0 everywhere, so Q[g] >= 0; in the first pass (the raw cumulative
       // charge may be negative) it is checked.  Every remaining pass ends at i* = nout-2 (no hit)
       // with e and g at their clamps, exactly as running it would; Q is not read again (the tape
       // follows Q0).
-      if (t == 0) {
+      double v;
+      if (held(e_none, v)) {
         for (int kk = k; kk < H; ++kk) {
-          int ee = nout - 2 + 1 + d.interval;
-          if (ee >= nout) ee = nout - 1;
-          int gg = nout - 2 + 1 + d.interval + 1;
-          if (gg >= nout) gg = nout - 1;
-          s_pe[kk] = ee;
-          s_pg[kk] = gg;
+          s_pe[kk] = e_none;
+          s_pg[kk] = g_none;
           s_pi[kk] = nout - 2;
-          s_pa[kk] = Q[ee];
+          s_pa[kk] = v;
         }
       }
       __syncthreads();
@@ -392,23 +417,27 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
     if (e >= nout) e = nout - 1;
     int g = istar + 1 + d.interval + 1;
     if (g >= nout) g = nout - 1;
-    const double Qg = Q[g];
-    if (t == 0) {
-      s_pe[k] = e;
-      s_pg[k] = g;
-      s_pi[k] = istar;
-      s_pa[k] = Q[e];
+    {
+      double v;
+      if (held(e, v)) {
+        s_pe[k] = e;
+        s_pg[k] = g;
+        s_pi[k] = istar;
+        s_pa[k] = v;
+      }
+      if (held(g, v)) s_qg = v;
     }
     __syncthreads();
+    const double Qg = s_qg;
 #pragma unroll
     for (int u = 0; u < TPER; ++u) {
-      const int o = t + NT * u;
-      if (o < nout) {
-        const double x = Q[o] - Qg;
-        Q[o] = x < 0 ? 0.0 : x;
+      if (o0 + u < nout) {
+        const double x = q[u] - Qg;
+        q[u] = x < 0 ? 0.0 : x;
       }
     }
-    __syncthreads();
+    const double x = nx - Qg;  // the next thread's first tick, reset as its owner resets it
+    nx = x < 0 ? 0.0 : x;
   }
   if (t >= kWave) return;
   // lane k < H: pass k's hit
@@ -452,10 +481,11 @@ __device__ void fee_row(const Dev& d, double* Q, const double* Q0, int r, int pl
   if (t == 0) tp->plane = plane;
 }
 
-// Sum of the per-wave partial rows, waveform store, float64 cumulative charge into Q and Q0.
+// Sum of the per-wave partial rows, waveform store, float64 cumulative charge: thread t holds
+// Q[t * TPER + u] in q[u] (consecutive ticks per thread); with a tape Q0 receives a copy in LDS.
 template <int NT>
-__device__ void finish_row(const Dev& d, float* acc, int nwaves, int rstride, double* Q, double* Q0, int r, const RowOut& out,
-                           double* shd) {
+__device__ void finish_row(const Dev& d, const float* acc, int nwaves, int rstride, double (&q)[kMaxOut / NT],
+                           double* Q0, int r, const RowOut& out, double* shd) {
   constexpr int TPER = kMaxOut / NT;
   const int t = threadIdx.x;
   float row[TPER];
@@ -480,16 +510,16 @@ __device__ void finish_row(const Dev& d, float* acc, int nwaves, int rstride, do
   }
   double tot;
   const double ex = block_exclusive_scan<double>(run, shd, tot);
-  __syncthreads();  // all partial rows read before Q overwrites the LDS
 #pragma unroll
-  for (int u = 0; u < TPER; ++u) {
-    const int o = t * TPER + u;
-    if (o < d.nout) {
-      Q[o] = ex + loc[u];
-      Q0[o] = ex + loc[u];
+  for (int u = 0; u < TPER; ++u) q[u] = ex + loc[u];
+  // the scan's barriers follow every read of the partial rows, which Q0 may now overwrite
+  if (out.tape) {
+#pragma unroll
+    for (int u = 0; u < TPER; ++u) {
+      const int o = t * TPER + u;
+      if (o < d.nout) Q0[o] = q[u];
     }
   }
-  __syncthreads();
 }
 
 // Contributions of one segment to one pixel row.  The segment's stripes landing on the row share
@@ -697,9 +727,9 @@ __device__ __forceinline__ void pixel_bins(const Dev& d, const Tables& tb, const
 }
 
 // Dynamic LDS bytes of the waveform kernel: the per-wave partial rows, reused afterwards for the
-// float64 cumulative charge (two copies).
+// unreset float64 cumulative charge of the tape.
 inline size_t rows_lds_bytes(int waves, int nout) {
-  const size_t acc = (size_t)waves * (nout + kRowPad) * 4, q = (size_t)2 * nout * 8;
+  const size_t acc = (size_t)waves * (nout + kRowPad) * 4, q = (size_t)nout * 8;
   return (acc > q ? acc : q);
 }
 
@@ -1080,10 +1110,10 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     }
   }
   __syncthreads();
-  double* Q = (double*)smem;
-  double* Q0 = Q + nout;
-  finish_row<NT>(d, acc, WAVES, rstride, Q, Q0, r, out, shd);
-  fee_row<NT>(d, Q, Q0, r, plane, out, shi);
+  double q[kMaxOut / NT];
+  double* Q0 = (double*)smem;
+  finish_row<NT>(d, acc, WAVES, rstride, q, Q0, r, out, shd);
+  fee_row<NT>(d, q, Q0, r, plane, out, shi);
 }
 
 // Self-trigger on caller waveforms (simulate_stochastic on an arbitrary wfs): one workgroup per
@@ -1091,7 +1121,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
 template <int NT>
 __global__ __launch_bounds__(NT) void k_fee_rows(Dev d, const float* __restrict__ wfs, const int64_t* __restrict__ pids,
                                                  int64_t n_rows, RowOut out) {
-  __shared__ __align__(16) char smem[2 * kMaxOut * 8];
+  __shared__ __align__(16) char smem[kMaxOut * 8];
   __shared__ float accs[kMaxOut];
   __shared__ double shd[NT / kWave];
   __shared__ int shi[NT / kWave];
@@ -1102,12 +1132,12 @@ __global__ __launch_bounds__(NT) void k_fee_rows(Dev d, const float* __restrict_
   const int64_t pid = pids[r];
   const int npl = d.nx * d.ny;
   const int plane = pid >= 0 ? (int)((pid / npl) % d.ntpc) : 0;
-  double* Q = (double*)smem;
-  double* Q0 = Q + d.nout;
+  double q[kMaxOut / NT];
+  double* Q0 = (double*)smem;
   RowOut o2 = out;
   o2.wfs = nullptr;
-  finish_row<NT>(d, accs, 1, d.nout, Q, Q0, (int)r, o2, shd);
-  fee_row<NT>(d, Q, Q0, (int)r, plane, o2, shi);
+  finish_row<NT>(d, accs, 1, d.nout, q, Q0, (int)r, o2, shd);
+  fee_row<NT>(d, q, Q0, (int)r, plane, o2, shi);
   if (threadIdx.x == 0 && pid < 0) out.hit_cnt[r] = 0;
 }
 
