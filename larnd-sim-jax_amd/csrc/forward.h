@@ -1007,12 +1007,16 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
     const int W4 = kWinPad * 4, nsb = d.nsb, side = 2 * d.ne + 1;
     const __amdgpu_buffer_rsrc_t tnp =
         __builtin_amdgcn_make_buffer_rsrc((void*)tb.TNP, 0, d.lx * d.ly * kWinPad * 4, kBufDesc3);
-    for (int b0 = wid * kWave; b0 < nfar; b0 += WAVES * kWave) {
+    // lane l's pair of the block at b0: its cell (a binary search of the cell list), record and
+    // correction.  The loads of the wave's next block are issued before this block's stores, so
+    // they arrive while the pairs are stored.
+    struct BlockIn {
+      PairRec P;
+      float D;
+      int sg, ab;
+    };
+    auto fetch = [&](int b0, BlockIn& B) {
       const int p = b0 + lane;
-      // padding lanes: an interior window at S = 2 with zero charge and correction (adds +0)
-      int rS = (2 << 1) | 1, roff = 0, rsg = 0, rab = 0;
-      float rso = 0.f, rsf = 0.f, rfD = 0.f, romfD = 0.f;
-      bool bins = false;
       if (p < nfar) {
         int lo = 0, hi = nsc - 1;  // the cell holding p: last c with c_cum[c] <= p
         while (lo < hi) {
@@ -1020,8 +1024,24 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
           if (c_cum[mid] <= p) lo = mid;
           else hi = mid - 1;
         }
-        const int sg = c_beg[lo] + (p - c_cum[lo]);
-        const int ab = c_ab[lo];
+        B.sg = c_beg[lo] + (p - c_cum[lo]);
+        B.ab = c_ab[lo];
+        const int a = (B.ab >> 8) - 64, b = (B.ab & 255) - 64;
+        B.P = prs[B.sg];
+        B.D = dtot[(int64_t)((a + d.ne) * side + (b + d.ne)) * nseg + B.sg];
+      }
+    };
+    BlockIn cur, nxt;
+    if (wid * kWave < nfar) fetch(wid * kWave, cur);
+    for (int b0 = wid * kWave; b0 < nfar; b0 += WAVES * kWave) {
+      if (b0 + WAVES * kWave < nfar) fetch(b0 + WAVES * kWave, nxt);
+      const int p = b0 + lane;
+      // padding lanes: an interior window at S = 2 with zero charge and correction (adds +0)
+      int rS = (2 << 1) | 1, roff = 0, rsg = 0, rab = 0;
+      float rso = 0.f, rsf = 0.f, rfD = 0.f, romfD = 0.f;
+      bool bins = false;
+      if (p < nfar) {
+        const int sg = cur.sg, ab = cur.ab;
         const int a = (ab >> 8) - 64, b = (ab & 255) - 64;
         // row offset = cell constant + ma U + za U0 + mb V + zb V0 (bytes)
         int kc = 0, ma = 0, za = 0, mb = 0, zb = 0;
@@ -1031,8 +1051,8 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         if (b > 0) { kc += (nsb * b + nsb / 2 - 1) * W4; mb = -1; }
         else if (b < 0) { kc += (nsb * -b - nsb / 2) * W4; mb = 1; }
         else zb = 1;
-        const PairRec P = prs[sg];
-        const float D = dtot[(int64_t)((a + d.ne) * side + (b + d.ne)) * nseg + sg];
+        const PairRec& P = cur.P;
+        const float D = cur.D;
         // with no neighbour stripes (nn = 0) a side pixel gets its diffusion bins only
         const float st = a >= -d.nn && a <= d.nn && b >= -d.nn && b <= d.nn ? 1.f : 0.f;
         roff = kc + ma * P.U + za * P.U0 + mb * P.V + zb * P.V0;
@@ -1093,6 +1113,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
         const int S = d.nt - d.L - R.k0;
         store_window(accw, lane, v0, v1, v2, 1.f, 1.f - R.f, R.f, 0.f, S, d.L, d.nout, S >= 2 && S + d.L <= d.nout);
       }
+      cur = nxt;
     }
   }
   // the centre cell: the window of each segment's bins on this pixel, from the interpolated
