@@ -437,6 +437,27 @@ def prob_fee_leg(params, wfs, steps, cpu=True, cpu_rows=16):
     leg = dict(rows=P, ticks=T, paths=V, hit_indices=S, steps_run=active, ms=ms, rows_per_s=P / (ms * 1e-3),
                path_ticks_per_s=P * active * V * (T - 1) / (ms * 1e-3),
                out_bytes=2 * P * S * (T - 1) * 4, out_gbs=2 * P * S * (T - 1) * 4 / (ms * 1e-3) / 1e9)
+    # forward + backward (lsim_prob_fee_backward, path selections held fixed) with cotangents on the
+    # probabilities and on the expected charges, the shape of the reference's probabilistic losses
+    wg = wfs.detach().clone().requires_grad_(True)
+    gen = torch.Generator(device=wfs.device).manual_seed(11)
+    Gp = torch.randn(lp.shape, generator=gen, device=wfs.device, dtype=lp.dtype)
+    Gq = torch.randn(q.shape, generator=gen, device=wfs.device, dtype=q.dtype) * 1e-4
+
+    def fwd_bwd():
+        wg.grad = None
+        lpg, qg = get_adc_values_average_noise_vmap(pp, wg)
+        ((Gp * lpg.exp()).sum() + (Gq * qg).sum()).backward()
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        fwd_bwd()
+    e1.record()
+    torch.cuda.synchronize()
+    leg["ms_fwd_bwd"] = e0.elapsed_time(e1) / steps
+    leg["rows_per_s_fwd_bwd"] = P / (leg["ms_fwd_bwd"] * 1e-3)
     if cpu:
         sys.path.insert(0, ROOT)
         from threadpoolctl import threadpool_limits
@@ -591,6 +612,9 @@ def main():
                     help="profiling: only the probabilistic readout on the 200k-segment event's rows")
     ap.add_argument("--batched-bwd-only", action="store_true",
                     help="profiling: only the 32-event batched forward+backward graph replays (no baseline)")
+    ap.add_argument("--single-only", action="store_true",
+                    help="profiling: only the 200k-segment event's legs (forward, forward+backward, finite "
+                         "differences, waveform cotangent)")
     ap.add_argument("--headline-only", action="store_true",
                     help="profiling: only the 128-event headline (no other leg, parity check or CPU baseline)")
     ap.add_argument("--profile-batch", action="store_true",
@@ -666,58 +690,63 @@ def main():
             log(f"[bench] profile forward {i + 1}/{a.steps}")
         return
 
-    # ---- headline: the batched forward of 128 uneven events, one graph replay per step ----
-    log("[bench] generating 128 events (log-uniform 2k-400k segments)")
-    bt, fields = synthetic.workload("c3")
-    sizes3 = synthetic.event_sizes("c3")
-    NB = bt.shape[0]
-    btr = torch.from_numpy(bt).to(dev)
-    brun = S.Runner(simr, params, btr, fields, n_events=128, want_wfs=True)
-    PB = brun.rows
-    log(f"[bench] N={NB} rows={PB}")
-    if a.waves:
+    if not a.single_only:
+        # ---- headline: the batched forward of 128 uneven events, one graph replay per step ----
+        log("[bench] generating 128 events (log-uniform 2k-400k segments)")
+        bt, fields = synthetic.workload("c3")
+        sizes3 = synthetic.event_sizes("c3")
+        NB = bt.shape[0]
+        btr = torch.from_numpy(bt).to(dev)
+        brun = S.Runner(simr, params, btr, fields, n_events=128, want_wfs=True)
+        PB = brun.rows
+        log(f"[bench] N={NB} rows={PB}")
+        if a.waves:
+            S.set_row_waves(a.waves)
+            S.set_row_cost(a.row_cost)
+            S.set_row_tile(a.row_tile)
+            waves, row_cost, row_tile, tune = a.waves, a.row_cost, a.row_tile, {}
+        else:
+            waves, row_cost, row_tile, tune = S.autotune_rows(brun)
+        log(f"[bench] waves per row workgroup: {waves}, row cost {row_cost} pairs, row tile {row_tile} "
+            f"(tuning ms: {tune})")
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            brun.forward()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            brun.forward()
+        torch.cuda.synchronize()
+        for _ in range(a.warmup):
+            g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ms_step = dt / a.steps * 1e3
+        seg_s = NB / (dt / a.steps)
+        # the waveform kernel's own time: HIP events around its launches on the launch stream, over
+        # eager forwards of the same batch right after the timed region
+        S.timing(True)
+        for _ in range(3):
+            brun.forward()
+        k_ms, k_n = S.timing_read()
+        S.timing(False)
+        k_avg = k_ms / max(k_n, 1)
+        bytes_fwd = algorithmic_bytes_fwd(NB, PB)
+        achieved = bytes_fwd / (k_avg * 1e-3) / 1e9
+        log(f"[bench] batched {NB} segments {ms_step:.2f} ms -> {seg_s:.3e} seg/s; waveform kernel {k_avg:.2f} ms")
+        del brun, btr, g
+
+    extra = {}
+    if a.single_only and a.waves:
         S.set_row_waves(a.waves)
         S.set_row_cost(a.row_cost)
         S.set_row_tile(a.row_tile)
-        waves, row_cost, row_tile, tune = a.waves, a.row_cost, a.row_tile, {}
-    else:
-        waves, row_cost, row_tile, tune = S.autotune_rows(brun)
-    log(f"[bench] waves per row workgroup: {waves}, row cost {row_cost} pairs, row tile {row_tile} "
-        f"(tuning ms: {tune})")
-    g = torch.cuda.CUDAGraph()
-    s = torch.cuda.Stream()
-    s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
-        brun.forward()
-    torch.cuda.current_stream().wait_stream(s)
-    torch.cuda.synchronize()
-    with torch.cuda.graph(g):
-        brun.forward()
-    torch.cuda.synchronize()
-    for _ in range(a.warmup):
-        g.replay()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        g.replay()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = dt / a.steps * 1e3
-    seg_s = NB / (dt / a.steps)
-    # the waveform kernel's own time: HIP events around its launches on the launch stream, over
-    # eager forwards of the same batch right after the timed region
-    S.timing(True)
-    for _ in range(3):
-        brun.forward()
-    k_ms, k_n = S.timing_read()
-    S.timing(False)
-    k_avg = k_ms / max(k_n, 1)
-    bytes_fwd = algorithmic_bytes_fwd(NB, PB)
-    achieved = bytes_fwd / (k_avg * 1e-3) / 1e9
-    log(f"[bench] batched {NB} segments {ms_step:.2f} ms -> {seg_s:.3e} seg/s; waveform kernel {k_avg:.2f} ms")
-    del brun, btr, g
-
-    extra = {}
     if a.headline_only:
         print(json.dumps({"metric": "segments/s", "value": seg_s, "ms_per_step": ms_step, "kernel_ms": k_avg,
                           "achieved_gbs": achieved, "config": {"segments": NB, "pixel_rows": PB, "row_waves": waves,
@@ -799,6 +828,9 @@ def main():
                                                    fd_speedup=dtfd / dtw)
         log(f"[bench] fwd+bwd with a waveform cotangent {dtw * 1e3:.3f} ms")
         del st_w, d_w
+    if a.single_only:
+        print(json.dumps(extra), flush=True)
+        return
 
     # ---- batched forward + backward: 32 uneven events, one graph replay, vs per-event FD ----
     if not a.no_batched_bwd:
