@@ -1,12 +1,16 @@
-"""A small read-only HDF5 reader for the reference's segment files (numpy only).
+"""A small HDF5 reader and writer for the reference's files (numpy only).
 
-The reference reads its inputs with h5py (``f['segments'][:]``, optimize/dataio.py:113-114), which
-is not installed here.  This module reads what those files use, following the HDF5 file-format
-specification (version 0 superblock, symbol-table groups with version 1 B-trees and local heaps,
-version 1 object headers with continuation blocks, version 1/2 dataspaces, fixed-point,
-floating-point and compound datatypes, contiguous and compact data layouts) and returns a numpy
-structured array.  Anything else (chunked or filtered datasets, other superblock versions, variable-length
-types) raises NotImplementedError instead of guessing.
+The reference reads its inputs with h5py (``f['segments'][:]``, optimize/dataio.py:113-114) and
+writes its hits as per-event groups (optimize/simulate.py:135-155); h5py is not installed here.
+This module follows the HDF5 file-format specification for what those files use: the version 0
+superblock, symbol-table groups (version 1 B-trees, symbol-table nodes, local heaps), version 1
+object headers with continuation blocks, version 1/2 dataspaces, fixed-point, floating-point and
+compound datatypes, and contiguous and compact layouts.  Reading returns numpy arrays (structured
+for a compound type); anything else (chunked or filtered datasets, other superblock versions,
+variable-length types) raises NotImplementedError instead of guessing.  Writing produces the
+same structures as the h5py files the reference ships (superblock version 0 with group node
+sizes K = 4 and 16, one symbol-table message per group, datasets with dataspace, datatype, fill
+value and contiguous layout messages) for groups of numeric datasets.
 """
 from __future__ import annotations
 
@@ -196,6 +200,175 @@ class _File:
         if n < count * dt.itemsize:
             raise ValueError("dataset storage smaller than its shape")
         return np.frombuffer(self.d, dtype=dt, count=count, offset=self.base + a).reshape(shape).copy()
+
+
+def _tree(f, addr):
+    out = {}
+    btree = heap = None
+    for mt, _, q, _ in f.messages(addr):
+        if mt == 0x11:
+            btree, heap = f.u("QQ", q)
+    if btree is None and addr == f.root_header:
+        btree, heap = f.root_btree, f.root_heap
+    for name, hdr in sorted(f.group_entries(btree, heap).items()):
+        is_group = any(mt == 0x11 for mt, _, _, _ in f.messages(hdr))
+        out[name] = _tree(f, hdr) if is_group else f.dataset(hdr)
+    return out
+
+
+def read_tree(filename):
+    """Every group and dataset of an HDF5 file as nested dicts {name: dict | numpy array}."""
+    with open(filename, "rb") as fh:
+        f = _File(fh.read())
+    return _tree(f, f.root_header)
+
+
+# ---- writing ----
+
+_LEAF_K, _NODE_K = 4, 16  # symbol-table node and group B-tree node sizes (the library's defaults)
+_SNOD_CAP, _BTREE_CAP = 2 * _LEAF_K, 2 * _NODE_K
+_FREE_NULL = 1  # end of a local heap's free list
+
+
+def _pad8(n):
+    return (n + 7) & ~7
+
+
+def _dtype_message(dt):
+    """Datatype message (version 1) of a little-endian numeric numpy dtype."""
+    dt = np.dtype(dt)
+    if dt.byteorder == ">" or (dt.byteorder == "=" and np.little_endian is False):
+        raise NotImplementedError("big-endian data")
+    if dt.kind in "iu":
+        bits = 0x08 if dt.kind == "i" else 0x00
+        return struct.pack("<BBBBIHH", 0x10, bits, 0, 0, dt.itemsize, 0, 8 * dt.itemsize)
+    if dt.kind == "f" and dt.itemsize in (4, 8):
+        if dt.itemsize == 4:
+            sign, exp_loc, exp_size, man_size, bias = 31, 23, 8, 23, 127
+        else:
+            sign, exp_loc, exp_size, man_size, bias = 63, 52, 11, 52, 1023
+        return struct.pack("<BBBBIHHBBBBI", 0x11, 0x20, sign, 0, dt.itemsize, 0, 8 * dt.itemsize, exp_loc, exp_size,
+                           0, man_size, bias)
+    raise NotImplementedError(f"HDF5 writing of dtype {dt}")
+
+
+class _Writer:
+    def __init__(self):
+        self.buf = bytearray(96)  # the superblock, written last
+
+    def alloc(self, n):
+        a = _pad8(len(self.buf))
+        self.buf.extend(b"\0" * (a + n - len(self.buf)))
+        return a
+
+    def put(self, addr, data):
+        self.buf[addr:addr + len(data)] = data
+
+    def header(self, messages):
+        """Version 1 object header holding [(type, flags, payload)]; returns its address."""
+        body = b""
+        for mt, fl, payload in messages:
+            payload = payload + b"\0" * (_pad8(len(payload)) - len(payload))
+            body += struct.pack("<HHB3x", mt, len(payload), fl) + payload
+        a = self.alloc(16 + len(body))
+        self.put(a, struct.pack("<BBHII4x", 1, 0, len(messages), 1, len(body)) + body)
+        return a
+
+    def dataset(self, arr):
+        arr = np.ascontiguousarray(arr)
+        if arr.dtype.byteorder == ">":
+            arr = arr.astype(arr.dtype.newbyteorder("<"))
+        shape = arr.shape if arr.ndim else (1,)
+        space = struct.pack("<BBBB4x", 1, len(shape), 1, 0) + struct.pack(f"<{len(shape)}Q", *shape) * 2
+        nbytes = arr.size * arr.dtype.itemsize
+        data = self.alloc(nbytes) if nbytes else _UNDEF
+        if nbytes:
+            self.put(data, arr.tobytes())
+        layout = struct.pack("<BBQQ", 3, 1, data, nbytes)
+        fill = struct.pack("<BBBBI", 2, 2, 2, 1, 0)
+        return self.header([(0x1, 0, space), (0x3, 1, _dtype_message(arr.dtype)), (0x5, 1, fill), (0x8, 0, layout)])
+
+    def group(self, tree):
+        """Objects of a group first, then its local heap, symbol-table nodes and B-tree; returns
+        (object header address, B-tree address, local heap address)."""
+        names = sorted(tree, key=lambda n: n.encode())
+        objs = {}
+        for n in names:
+            v = tree[n]
+            objs[n] = self.group(v) if isinstance(v, dict) else (self.dataset(np.asarray(v)), None, None)
+        # local heap: "" at offset 0, the names, then one free block (as the library leaves it)
+        heap_data = bytearray(8)
+        offs = {}
+        for n in names:
+            b = n.encode()
+            if b"\0" in b or b"/" in b or not b:
+                raise ValueError(f"bad HDF5 link name {n!r}")
+            offs[n] = len(heap_data)
+            heap_data += b + b"\0" * (_pad8(len(b) + 1) - len(b))
+        free = len(heap_data)
+        heap_data += struct.pack("<QQ", _FREE_NULL, 16)
+        heap = self.alloc(32)
+        data = self.alloc(len(heap_data))
+        self.put(data, bytes(heap_data))
+        self.put(heap, b"HEAP" + struct.pack("<B3xQQQ", 0, len(heap_data), free, data))
+        # symbol-table nodes of at most 2 K_leaf entries, in name order
+        leaves = []
+        for i in range(0, len(names), _SNOD_CAP):
+            chunk = names[i:i + _SNOD_CAP]
+            a = self.alloc(8 + _SNOD_CAP * 40)
+            ent = b""
+            for n in chunk:
+                hdr, bt, hp = objs[n]
+                if bt is None:
+                    ent += struct.pack("<QQI4x16x", offs[n], hdr, 0)
+                else:
+                    ent += struct.pack("<QQI4xQQ", offs[n], hdr, 1, bt, hp)
+            self.put(a, b"SNOD" + struct.pack("<BxH", 1, len(chunk)) + ent)
+            leaves.append((a, offs[chunk[-1]]))
+        # B-tree levels of at most 2 K_node children; key i + 1 is the last name under child i
+        level, nodes = 0, leaves
+        while True:
+            parents = []
+            for i in range(0, max(len(nodes), 1), _BTREE_CAP):
+                chunk = nodes[i:i + _BTREE_CAP]
+                a = self.alloc(24 + (2 * _BTREE_CAP + 1) * 8)
+                body = struct.pack("<Q", 0)
+                for child, last in chunk:
+                    body += struct.pack("<QQ", child, last)
+                self.put(a, b"TREE" + struct.pack("<BBHQQ", 0, level, len(chunk), _UNDEF, _UNDEF) + body)
+                parents.append((a, chunk[-1][1] if chunk else 0))
+            for (a, _), (b, _) in zip(parents, parents[1:]):  # sibling links of the level
+                self.put(a + 16, struct.pack("<Q", b))
+                self.put(b + 8, struct.pack("<Q", a))
+            if len(parents) == 1:
+                btree = parents[0][0]
+                break
+            level, nodes = level + 1, parents
+        hdr = self.header([(0x11, 0, struct.pack("<QQ", btree, heap))])
+        return hdr, btree, heap
+
+    def finish(self, root):
+        hdr, btree, heap = root
+        eof = _pad8(len(self.buf))
+        self.buf.extend(b"\0" * (eof - len(self.buf)))
+        sb = _SIG + struct.pack("<8B", 0, 0, 0, 0, 0, 8, 8, 0) + struct.pack("<HHI", _LEAF_K, _NODE_K, 0)
+        sb += struct.pack("<QQQQ", 0, _UNDEF, eof, _UNDEF)
+        sb += struct.pack("<QQI4xQQ", 0, hdr, 1, btree, heap)
+        self.put(0, sb)
+        return bytes(self.buf)
+
+
+def write_tree(filename, tree):
+    """Write nested dicts {name: dict (a group) | numeric numpy array (a dataset)} as an HDF5 file
+    (replacing it); read_tree(filename) gives the same tree back."""
+    w = _Writer()
+    data = w.finish(w.group(tree))
+    tmp = str(filename) + ".tmp"
+    with open(tmp, "wb") as fh:
+        fh.write(data)
+    import os
+
+    os.replace(tmp, filename)
 
 
 def read_dataset(filename, path):

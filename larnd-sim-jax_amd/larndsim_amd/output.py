@@ -2,8 +2,8 @@
 HDF5 group layout, and the single npz file.
 
 Host-side bookkeeping after the GPU readout: the hit arrays are copied to the host once and
-written.  h5py is optional; without it ``write_hdf5`` raises ImportError and the npz writer
-still works.
+written.  The HDF5 file is written by the package's own writer (``h5lite``), in the layout
+h5py gives the reference's files; no h5py is needed.
 """
 from __future__ import annotations
 
@@ -56,17 +56,27 @@ def event_groups(hits, local_to_global=None):
 
 
 def write_hdf5(path, batch_index, groups):
-    """Append ``batch_<i>/event_<g>/<dataset>`` groups to an HDF5 file (simulate.py:135-155)."""
-    try:
-        import h5py
-    except ImportError as e:  # pragma: no cover - depends on the installation
-        raise ImportError("write_hdf5 needs h5py; use save_npz for the npz output") from e
-    with h5py.File(path, "a") as f:
-        b = f.create_group(f"batch_{batch_index}")
-        for name, ds in groups.items():
-            g = b.create_group(name)
-            for k, v in ds.items():
-                g.create_dataset(k, data=v)
+    """Append ``batch_<i>/event_<g>/<dataset>`` groups to an HDF5 file (simulate.py:135-155, which
+    opens the file in append mode and creates one batch group per call).  The file's groups are
+    read back and the whole file is rewritten with the new batch; like h5py's create_group, an
+    existing batch group of the same name raises ValueError."""
+    import os
+
+    from . import h5lite
+
+    tree = h5lite.read_tree(path) if os.path.exists(path) else {}
+    name = f"batch_{batch_index}"
+    if name in tree:
+        raise ValueError(f"{path}: group {name} already exists")
+    tree[name] = {ev: {k: np.asarray(v) for k, v in ds.items()} for ev, ds in groups.items()}
+    h5lite.write_tree(path, tree)
+
+
+def read_hdf5(path):
+    """The groups of a hit file as nested dicts {batch: {event: {dataset: array}}}."""
+    from . import h5lite
+
+    return h5lite.read_tree(path)
 
 
 def save_npz(path, batches):

@@ -52,9 +52,25 @@ def test_npz_round_trip(params, tmp_path):
     assert np.array_equal(z_["z"], np.concatenate([h1["pix_z"], h2["pix_z"]]))
 
 
-def test_hdf5_needs_h5py(tmp_path):
-    try:
-        import h5py  # noqa: F401
-    except ImportError:
-        with pytest.raises(ImportError):
-            output.write_hdf5(tmp_path / "x.h5", 0, {})
+def test_hdf5_batches_round_trip(params, tmp_path):
+    """write_hdf5 appends batch_<i>/event_<g>/<dataset> groups (simulate.py:135-155); the file reads
+    back with every dataset's values and dtype, and a second write of the same batch raises as
+    h5py's create_group does."""
+    adcs, x, y, z, ticks, prob, event, pid = _readout(params)
+    h = output.select_hits(params, adcs, x, y, z, ticks, prob, event, pid)
+    g0 = output.event_groups(h, {0: 1005, 1: 2000})
+    g1 = output.event_groups(h, {0: 7, 1: 8})
+    path = tmp_path / "hits.h5"
+    output.write_hdf5(path, 0, g0)
+    output.write_hdf5(path, 1, g1)
+    back = output.read_hdf5(path)
+    assert sorted(back) == ["batch_0", "batch_1"]
+    assert sorted(back["batch_1"]) == ["event_7", "event_8"]
+    for b, g in (("batch_0", g0), ("batch_1", g1)):
+        for ev, ds in g.items():
+            assert set(back[b][ev]) == set(output.EVENT_KEYS)
+            for k, v in ds.items():
+                got = back[b][ev][k]
+                assert got.dtype == np.asarray(v).dtype and np.array_equal(got, v), (b, ev, k)
+    with pytest.raises(ValueError):
+        output.write_hdf5(path, 1, g1)
