@@ -474,6 +474,58 @@ def prob_fee_leg(params, wfs, steps, cpu=True, cpu_rows=16):
     return leg
 
 
+def param_leg(params, trk, fields, steps, cpu=True, cpu_segments=10000):
+    """The parametrized current model (simulate_parametrized, sim_jax.py:339-372; mc_diff, one
+    electron per segment, diffusion in the current, 4 neighbour pixels) on the 200k-segment
+    event, inputs resident: the whole call (electrons, pixel set, waveforms, readout, hits) by
+    events on torch's current stream, and the waveform kernel alone.  CPU baseline: the float64
+    NumPy oracle on the first cpu_segments segments, 1 thread."""
+    import numpy as np
+    import torch
+
+    from larndsim_amd import param as GP
+
+    p = params.replace(mc_diff=True, diffusion_in_current_sim=True)
+    out = GP.simulate_parametrized(p, trk, fields, rngseed=1)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        out = GP.simulate_parametrized(p, trk, fields, rngseed=1)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    e = GP.electrons(p, trk, fields, rngseed=1)
+    GP.waveforms(p, e)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        wfs, U = GP.waveforms(p, e)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_w = e0.elapsed_time(e1) / steps
+    n = int(trk.shape[0])
+    K = (2 * int(p.number_pix_neighbors) + 1) ** 2
+    leg = dict(workload=f"200k-segment event, mc_diff, {K} pixels per electron, 51 current samples", segments=n,
+               pairs=n * K, rows=int(U.numel()), hits=int(out[0].numel()), ms=ms, segments_per_s=n / (ms * 1e-3),
+               waveforms_ms=ms_w, pairs_per_s=n * K / (ms_w * 1e-3))
+    if cpu:
+        sys.path.insert(0, ROOT)
+        from threadpoolctl import threadpool_limits
+
+        from oracle import param as PA
+
+        tr = trk[:cpu_segments].cpu().numpy()
+        normals = np.random.default_rng(1).normal(size=(len(tr), 3))
+        with threadpool_limits(limits=1), np.errstate(all="ignore"):
+            t0 = time.perf_counter()
+            PA.simulate_parametrized(p, tr, fields, normals=normals)
+            dt = time.perf_counter() - t0
+        leg["cpu_baseline"] = dict(value=cpu_segments / dt, unit="segments/s", cores=1, kind="port",
+                                   sample=f"first {cpu_segments} segments, float64 NumPy oracle, 1 thread, {dt:.2f} s")
+    return leg
+
+
 def sdtw_leg(params, lut, trk, fields, steps, cpu=True):
     """Soft-DTW loss (value + gradient) between the hit ADC sequences of the 200k-segment event at
     nominal and at perturbed parameters, inputs resident; events around the whole lsim_sdtw call.
@@ -607,6 +659,9 @@ def main():
     ap.add_argument("--chop-only", action="store_true", help="profiling: only the track-chopping leg")
     ap.add_argument("--no-prob-fee", action="store_true", help="skip the probabilistic readout leg")
     ap.add_argument("--no-sdtw", action="store_true", help="skip the soft-DTW leg")
+    ap.add_argument("--no-param", action="store_true", help="skip the parametrized current model leg")
+    ap.add_argument("--param-only", action="store_true",
+                    help="profiling: only the parametrized current model on the 200k-segment event")
     ap.add_argument("--sdtw-only", action="store_true", help="profiling: only the soft-DTW leg")
     ap.add_argument("--prob-fee-only", action="store_true",
                     help="profiling: only the probabilistic readout on the 200k-segment event's rows")
@@ -634,7 +689,7 @@ def main():
         print(json.dumps(record_counters(a.record_counters)["derived"]))
         return
     if a.headline_only:
-        a.no_bwd = a.no_batched_bwd = a.no_prob_fee = a.no_sdtw = a.no_chop = a.no_cpu = True
+        a.no_bwd = a.no_batched_bwd = a.no_prob_fee = a.no_sdtw = a.no_chop = a.no_cpu = a.no_param = True
         a.fit_steps = a.fit_wfs_steps = 0
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (events would shard across "
@@ -670,6 +725,10 @@ def main():
     if a.sdtw_only:
         tracks, fields = synthetic.workload("c2")
         print(json.dumps(sdtw_leg(params, lut, torch.from_numpy(tracks).to(dev), fields, a.steps, cpu=not a.no_cpu)))
+        return
+    if a.param_only:
+        tracks, fields = synthetic.workload("c2")
+        print(json.dumps(param_leg(params, torch.from_numpy(tracks).to(dev), fields, a.steps, cpu=not a.no_cpu)))
         return
     if a.prob_fee_only:
         tracks, fields = synthetic.workload("c2")
@@ -844,6 +903,13 @@ def main():
                                          cpu=not a.no_cpu)
         log(f"[bench] prob fee {extra['prob_fee']['rows']} rows: {extra['prob_fee']['ms']:.2f} ms")
         del wst
+
+    # ---- the parametrized current model on the same event ----
+    if not a.no_param:
+        log("[bench] parametrized current model")
+        extra["param"] = param_leg(params, trk, fields, max(3, a.steps // 4), cpu=not a.no_cpu)
+        log(f"[bench] param {extra['param']['segments']} segments: {extra['param']['ms']:.2f} ms "
+            f"(waveforms {extra['param']['waveforms_ms']:.2f} ms)")
 
     # ---- soft-DTW fit loss on the event's hit sequences ----
     if not a.no_sdtw:
