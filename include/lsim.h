@@ -271,7 +271,7 @@ int lsim_param_electrons(lsim_ctx *c, const lsim_params *p, const lsim_segments 
  * linspace(0, 5) us, at waveform ticks t0_tick - samples + k (ticks outside the waveform are dropped, as
  * the reference's column 0 is).  diffusion != 0: the exponentially modified Gaussian tails of
  * current_model_diff with the electron's spread.  Rows with a negative id stay zero.  Float64 sums in a
- * fixed order (deterministic). */
+ * fixed order (deterministic).  The sample count must be 2 .. 63 (LSIM_E_ARG otherwise). */
 int lsim_param_rows(lsim_ctx *c, const lsim_params *p, const double *ex, const double *ey, const double *et0,
                     const double *ene, const double *esig, const int64_t *pixel_ids, int64_t n_rows,
                     const int32_t *pair_order, const int32_t *row_offsets, int32_t diffusion, float *wfs, void *stream);

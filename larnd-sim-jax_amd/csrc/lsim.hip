@@ -926,7 +926,7 @@ int lsim_param_rows(lsim_ctx* c, const lsim_params* p, const double* ex, const d
   int rc = make_dev(c, p, 1, d);
   if (rc) return rc;
   const int T = (int)(5.0 / d.ts) + 1;  // linspace(0, 5, int(5 / t_sampling) + 1)
-  if (T < 2 || T > kParamMaxSamples) return fail(LSIM_E_ARG, "t_sampling must give 2 .. 64 current samples in 5 us");
+  if (T < 2 || T > kParamMaxSamples) return fail(LSIM_E_ARG, "t_sampling must give 2 .. 63 current samples in 5 us");
   if (d.nout > kMaxOut) return fail(LSIM_E_ARG, "too many output ticks");
   if (n_rows == 0) return LSIM_OK;
   if (!ex || !ey || !et0 || !ene || !esig || !pixel_ids || !pair_order || !row_offsets || !wfs)

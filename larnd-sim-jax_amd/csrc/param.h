@@ -17,7 +17,7 @@
 
 namespace lsim {
 
-constexpr int kParamMaxSamples = 64;  // samples of one pair's current (one per lane)
+constexpr int kParamMaxSamples = kWave - 1;  // samples of one pair's current: one per lane, and a lane for the last tick boundary
 constexpr int kParamWaves = 4;
 constexpr int kParamMaxEdges = 17;    // transverse bin edges (16 bins)
 
@@ -100,21 +100,22 @@ __device__ __forceinline__ double emg_cdf_half(double X, double mu, double sigma
   return 0.5 * erf((X - mu) / (M_SQRT2 * sigma)) - pdf / lambd;
 }
 
-// One wave: this lane's sample k < T of one tail (decay `scale`) of a pair's current, sample times
-// x = -t_k, loc = -t0 shifted: integrated_expon_diff (diffusion) or integrated_expon.
-__device__ __forceinline__ double param_tail(double xk, double loc, double scale, double sigma, double dt, int T,
-                                             bool diffusion, int lane) {
+// One wave: sample k < T of one tail (decay `scale`) of a pair's current, loc = -t0 shifted,
+// integrated over the tick around sample time t_k: the integral runs between the boundaries
+// g_{k+1} = -t_k - dt/2 and g_k = -t_k + dt/2 (g_j = -t_j + dt/2 for j < T, g_T = -t_{T-1} - dt/2),
+// so lane j evaluates the tail's CDF (integrated_expon_diff, with diffusion) or its exponential
+// (integrated_expon) once at g_j and each value serves two samples (the next lane's by a shift).
+__device__ __forceinline__ double param_tail(double gj, double loc, double scale, double sigma, double dt, int T,
+                                             bool diffusion) {
   if (diffusion) {
-    const double lambd = 1.0 / scale;
-    const double up = emg_cdf_half(xk + dt / 2, loc, sigma, lambd);
-    const double lo = emg_cdf_half(xk - dt / 2, loc, sigma, lambd);
-    const double lo0 = __shfl(lo, 0, kWave), upT = __shfl(up, T - 1, kWave);
-    return (up - lo) / (lo0 - upT) / dt;
+    const double F = emg_cdf_half(gj, loc, sigma, 1.0 / scale);
+    const double Fn = __shfl_down(F, 1, kWave);
+    const double F1 = __shfl(F, 1, kWave), FT = __shfl(F, T - 1, kWave);  // lo of sample 0, up of T - 1
+    return (F - Fn) / (F1 - FT) / dt;
   }
-  (void)lane;
-  return (exp(fmin(0.0, (loc - xk + dt / 2) / scale)) - exp(fmin(0.0, (loc - xk - dt / 2) / scale)) +
-          exp(fmin(0.0, (loc - dt / 2) / scale)) / T) /
-         dt;
+  const double G = exp(fmin(0.0, (loc - gj) / scale));
+  const double Gn = __shfl_down(G, 1, kWave);
+  return (Gn - G + exp(fmin(0.0, (loc - dt / 2) / scale)) / T) / dt;
 }
 
 // One workgroup per row (sorted unique pixel id upix[r]): the pairs order[offs[r] .. offs[r+1]) in
@@ -142,6 +143,7 @@ __global__ __launch_bounds__(kParamWaves* kWave) void k_param_rows(Dev d, ParamE
     const double dt = 5.0 / (double)(T - 1);
     const int kk = lane < T ? lane : T - 1;
     const double tk = kk == T - 1 ? 5.0 : (double)kk * dt;  // linspace(0, 5, T)
+    const double gj = lane < T ? -tk + dt / 2 : -5.0 - dt / 2;  // tick boundaries (lane T: the last)
     const int b0 = offs[r], b1 = offs[r + 1];
     for (int i = b0 + wid; i < b1; i += kParamWaves) {
       const int pr = order[i];
@@ -158,8 +160,8 @@ __global__ __launch_bounds__(kParamWaves* kWave) void k_param_rows(Dev d, ParamE
       const int t0_tick = (int)(t0 / d.ts + 0.5);
       const double st0 = (t0 - t0_tick * d.ts) + ts0;
       const double sig = e.sig[el];
-      const double cur = a * param_tail(-tk, -st0, b, sig, dt, T, diffusion != 0, lane) +
-                         (1 - a) * param_tail(-tk, -st0, c, sig, dt, T, diffusion != 0, lane);
+      const double cur = a * param_tail(gj, -st0, b, sig, dt, T, diffusion != 0) +
+                         (1 - a) * param_tail(gj, -st0, c, sig, dt, T, diffusion != 0);
       const int o = t0_tick - T + kk;
       if (lane < T && o >= 0 && o < nout) acc[wid][o] += cur * e.ne[el];
     }
