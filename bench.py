@@ -206,17 +206,21 @@ def parity_c1(simr, params, lut):
                 hits=nh, hits_oracle=int(len(ref["adcs"])), hit_coordinate_max_abs_err_cm=coords)
 
 
+# The sources the forward kernels (k_rows_fwd, k_records and what they call) are built from, with
+# the library file that launches them; the other kernels' headers do not change them.
+FORWARD_SOURCES = ("device.h", "forward.h", "scan_sort.h", "lsim.hip")
+
+
 def kernel_source_stamp() -> str:
-    """sha256 (first 16 hex) of the HIP sources: ties a recorded counter measurement to the kernels
-    it measured."""
+    """sha256 (first 16 hex) of the forward kernels' sources: ties a recorded counter measurement
+    to the kernels it measured."""
     import hashlib
 
     d = os.path.join(ROOT, "larnd-sim-jax_amd", "csrc")
     h = hashlib.sha256()
-    for name in sorted(os.listdir(d)):
-        if name.endswith((".h", ".hip")):
-            with open(os.path.join(d, name), "rb") as f:
-                h.update(name.encode() + b"\0" + f.read())
+    for name in FORWARD_SOURCES:
+        with open(os.path.join(d, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
     return h.hexdigest()[:16]
 
 
