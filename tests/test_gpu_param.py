@@ -22,10 +22,11 @@ CASES = [  # (mc_diff, number_pix_neighbors, diffusion in the current)
     (True, 2, True),  # 22 of 1243 rows meet an overflowing pair, the same rows on both sides
     (False, 0, True),  # transverse bins: 25 electrons per segment (the central pixel never overflows)
     (True, 1, False),  # no diffusion: one-sided exponentials, finite everywhere
+    (True, 4, True),  # the neighbourhood of the benchmark (81 pixels per electron)
 ]
 
 
-@pytest.mark.parametrize("mc,nn,diffusion", CASES, ids=["mc-n0", "mc-n2", "bins-n0", "mc-n1-nodiff"])
+@pytest.mark.parametrize("mc,nn,diffusion", CASES, ids=["mc-n0", "mc-n2", "bins-n0", "mc-n1-nodiff", "mc-n4"])
 def test_parametrized_matches_oracle(params4, c1, mc, nn, diffusion):
     from larndsim_amd import param as GP
     from larndsim_amd.sim import get_simulator
