@@ -530,7 +530,10 @@ __device__ void finish_row(const Dev& d, const float* acc, int nwaves, int rstri
 // are kept for 1 <= o <= nout-1, corrections for 0 <= o <= nout-2 (the reference's garbage
 // column takes the rest).  Every per-pair quantity is wave uniform (scalar registers).
 constexpr int kChunk = kWave - 1;
-constexpr int kRowPad = 3 * kChunk;  // slack after each wave's partial row for the interior store
+// Slack after each wave's partial row for the interior store: a window at S (2 <= S, S + L <= nout)
+// is stored as three chunks of 63 lanes from index S - 2, so it writes up to S + 3 * 63 - 3 <=
+// nout - L + 186.
+__host__ __device__ inline int row_pad(int L) { return L < 3 * kChunk - 2 ? 3 * kChunk - 2 - L : 0; }
 
 // lane i receives lane i+1's value (lane 63 receives 0): DPP wavefront shift, no LDS traffic
 __device__ __forceinline__ float lane_next(float v) {
@@ -561,7 +564,7 @@ __device__ __forceinline__ void store_window(float* accw, int lane, float v0, fl
   const float n0 = lane_next(v0), n1 = lane_next(v1), n2 = lane_next(v2);
   if (interior) {
     // all three chunks in one go (disjoint addresses); lanes past the window add +0 to the
-    // row or to the kRowPad padding after it, so no per-chunk branch is needed
+    // row or to the row_pad slack after it, so no per-chunk branch is needed
     const float so = scale * omf, sf = scale * f;
     float* p = accw + (S - 2) + lane;
     float c0 = so * v0 + sf * n0;
@@ -726,11 +729,15 @@ __device__ __forceinline__ void pixel_bins(const Dev& d, const Tables& tb, const
   }
 }
 
-// Dynamic LDS bytes of the waveform kernel: the per-wave partial rows, reused afterwards for the
-// unreset float64 cumulative charge of the tape.
-inline size_t rows_lds_bytes(int waves, int nout) {
-  const size_t acc = (size_t)waves * (nout + kRowPad) * 4, q = (size_t)nout * 8;
-  return (acc > q ? acc : q);
+// Dynamic LDS of the waveform kernel: the per-wave partial rows (reused afterwards for the unreset
+// float64 cumulative charge of the tape), then the neighbourhood's cell lists (ncell begins, ncell + 1
+// cumulative counts, ncell offsets).
+__host__ __device__ inline size_t rows_acc_bytes(int waves, int nout, int L) {
+  const size_t acc = (size_t)waves * (nout + row_pad(L)) * 4, q = (size_t)nout * 8;
+  return ((acc > q ? acc : q) + 15) & ~(size_t)15;
+}
+inline size_t rows_lds_bytes(int waves, int nout, int L, int ncell) {
+  return rows_acc_bytes(waves, nout, L) + (size_t)(3 * ncell + 1) * 4;
 }
 
 constexpr int kMaxCells = 17 * 17;  // (2n+1)^2 for number_pix_neighbors <= 8
@@ -954,8 +961,6 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   extern __shared__ __align__(16) char smem[];
   __shared__ double shd[NT / kWave];
   __shared__ int shi[NT / kWave];
-  // this row's neighbourhood cells: far cells first, then the 3x3 near cells, in (a, b) order
-  __shared__ int c_beg[kMaxCells], c_cum[kMaxCells + 1], c_ab[kMaxCells];
   __shared__ int ncls[3];  // far cells, + the centre cell, + the 8 side cells
 
   const int P0 = *n_rows_dev;
@@ -966,10 +971,15 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, co
   const int r = order[slot];
   const int nout = d.nout;
   float* acc = (float*)smem;
+  // this row's neighbourhood cells: far cells first, then the 3x3 near cells, in (a, b) order
+  const int ncell = (2 * d.ne + 1) * (2 * d.ne + 1);
+  int* const c_beg = (int*)(smem + rows_acc_bytes(WAVES, nout, d.L));
+  int* const c_cum = c_beg + ncell;
+  int* const c_ab = c_cum + ncell + 1;
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1);
   const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
-  const int rstride = nout + kRowPad;
+  const int rstride = nout + row_pad(d.L);
   const int cell = row_cell[r];
   const int evp = cell / (d.gx * d.gy);  // ev * ntpc + plane
   const int plane = evp % d.ntpc;

@@ -286,7 +286,7 @@ int launch_row_bands(const Dev& d, const Ws& w, int64_t max_rows, hipStream_t st
 template <int NB, int WAVES>
 void launch_rows_nb(const Dev& d, const Tables& tb, const Ws& w, int64_t n_seg, int64_t max_rows, const RowOut& ro,
                     hipStream_t st) {
-  const size_t lds = rows_lds_bytes(WAVES, d.nout);
+  const size_t lds = rows_lds_bytes(WAVES, d.nout, d.L, (2 * d.ne + 1) * (2 * d.ne + 1));
   const int64_t grid = kXcd * band_slots(max_rows);
   hipLaunchKernelGGL((k_rows_fwd<NB, WAVES>), dim3((unsigned)grid), dim3(WAVES * kWave), lds, st, d, tb, w.rec, w.rng,
                      w.row_cell, w.n_rows, max_rows, w.band, row_order(w, d), w.dtot, w.prs, n_seg, ro);
