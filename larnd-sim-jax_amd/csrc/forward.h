@@ -951,7 +951,7 @@ constexpr int kFarGroup = LSIM_FAR_GROUP;  // independent neighbour-only pairs i
 // time), and computes row order[slot].  The grid holds kXcd times the
 // longest band allowed (rows_grid); the workgroups past their band's end exit at once.
 template <int NB, int WAVES>
-__global__ __launch_bounds__(WAVES * kWave) void k_rows_fwd(Dev d, Tables tb, const SegRec* __restrict__ rec,
+__global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(WAVES == 2 ? 5 : 1))) void k_rows_fwd(Dev d, Tables tb, const SegRec* __restrict__ rec,
                                                             const int2* __restrict__ rng, const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
                                                             const int* __restrict__ band, const int* __restrict__ order,
