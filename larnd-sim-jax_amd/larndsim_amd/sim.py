@@ -531,12 +531,13 @@ def autotune_row_waves(runner: Runner, candidates=(1, 2, 4, 8), reps=3):
     return best, res
 
 
-def autotune_rows(runner: Runner, waves=(1, 2, 4, 8), costs=(64, 256, 1024), tiles=(1, 8, 16, 32), reps=3):
+def autotune_rows(runner: Runner, waves=(1, 2, 4, 8), costs=(64, 256, 1024), tiles=(8, 16, 32), reps=3):
     """Tune the waveform kernel on the runner's workload, one knob at a time: waves per row
     workgroup (at the default row cost and tile), then the per-row cost the XCD bands are balanced
-    with, then the side of the cell tiles the rows are processed in.  All three only change the
-    schedule and the per-wave partial sums; the settings stay in effect.  Returns (waves, row cost,
-    tile, {setting: kernel ms})."""
+    with, then the side of the cell tiles the rows are processed in (tiled orders only: they time
+    within noise of the plain row order and fetch a third less from HBM).  All three only change
+    the schedule and the per-wave partial sums; the settings stay in effect.  Returns (waves, row
+    cost, tile, {setting: kernel ms})."""
     set_row_cost(256)
     set_row_tile(16)
     w, res_w = autotune_row_waves(runner, waves, reps)
