@@ -276,6 +276,26 @@ int lsim_param_rows(lsim_ctx *c, const lsim_params *p, const double *ex, const d
                     const double *ene, const double *esig, const int64_t *pixel_ids, int64_t n_rows,
                     const int32_t *pair_order, const int32_t *row_offsets, int32_t diffusion, float *wfs, void *stream);
 
+/* Reverse-mode derivative of the parametrized model (what jax.value_and_grad takes through
+ * ParametrizedSimulation.predict, optimize/strategies.py:136-149 -> simulate_parametrized,
+ * src/larndsim/sim_jax.py:339-372): d loss / d (Ab, kb, eField, lifetime, vdrift, long_diff, tran_diff)
+ * into d_params [7] float64 (device), from cotangents on the readout grid d_adc [n_rows][H] and
+ * d_z [n_rows][H] (may be NULL) and on the waveforms d_wfs [n_rows][n_ticks - 1] (may be NULL).
+ * The arguments are those of the forward: the segments, mode, diffusion, normals and bin_edges given
+ * to lsim_param_electrons and the electrons it wrote; the rows given to lsim_param_rows; fee_tape, the
+ * tape lsim_fee_forward wrote for those rows' waveforms (out->tape).  Pixel ids, the whole-tick split
+ * of the arrival time and the readout's decisions are held at their forward values.  Ticks whose
+ * cotangent is exactly zero contribute nothing (the reference propagates a NaN of an overflowing
+ * pair there).  Deterministic.  lsim_param_backward_query: workspace bytes for E electrons and
+ * n_rows rows. */
+int lsim_param_backward_query(lsim_ctx *c, int64_t n_electrons, int64_t n_rows, size_t *ws_bytes);
+int lsim_param_backward(lsim_ctx *c, const lsim_params *p, const lsim_segments *seg, int32_t mode, int32_t diffusion,
+                        const float *normals, const double *bin_edges, int32_t n_edges, const double *ex,
+                        const double *ey, const double *et0, const double *ene, const double *esig,
+                        const int64_t *pixel_ids, int64_t n_rows, const int32_t *pair_order, const int32_t *row_offsets,
+                        const void *fee_tape, const float *d_adc, const float *d_z, const float *d_wfs, void *ws,
+                        size_t ws_bytes, double *d_params, void *stream);
+
 /* The distance beyond which lsim_rbf_sums_grid leaves pairs out (1.01 sqrt(212) sigma); callers size
  * the grid's cells from it (any cell size is correct; about a third of it culls well). */
 float lsim_rbf_cull_radius(float sigma);

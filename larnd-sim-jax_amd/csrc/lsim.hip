@@ -940,6 +940,71 @@ int lsim_param_rows(lsim_ctx* c, const lsim_params* p, const double* ex, const d
   return LSIM_OK;
 }
 
+int lsim_param_backward_query(lsim_ctx* c, int64_t n_electrons, int64_t n_rows, size_t* bytes) {
+  if (!c || !bytes) return fail(LSIM_E_ARG, "NULL argument");
+  if (n_electrons < 0 || n_rows < 0) return fail(LSIM_E_ARG, "negative size");
+  Carve cv(nullptr);
+  cv.take<ParamAdj>(n_electrons);
+  cv.take<Pieces>(n_rows);
+  cv.take<double>(n_rows);
+  cv.take<double>(n_rows * kNGrad);
+  cv.take<int64_t>(1);
+  *bytes = cv.off + 256;
+  return LSIM_OK;
+}
+
+int lsim_param_backward(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t mode, int32_t diffusion,
+                        const float* normals, const double* bin_edges, int32_t n_edges, const double* ex,
+                        const double* ey, const double* et0, const double* ene, const double* esig,
+                        const int64_t* pixel_ids, int64_t n_rows, const int32_t* pair_order, const int32_t* row_offsets,
+                        const void* fee_tape, const float* d_adc, const float* d_z, const float* d_wfs, void* ws,
+                        size_t ws_bytes, double* d_params, void* stream) {
+  if (!c || !p || !seg || !d_params || !ws) return fail(LSIM_E_ARG, "NULL argument");
+  if (mode != 0 && mode != 1) return fail(LSIM_E_ARG, "mode must be 0 (transverse bins) or 1 (normal draws)");
+  if (mode == 0 && (n_edges < 2 || n_edges > kParamMaxEdges || !bin_edges))
+    return fail(LSIM_E_ARG, "transverse bins need 2 .. 17 bin edges");
+  if (mode == 1 && seg->n > 0 && !normals) return fail(LSIM_E_ARG, "normal draws are NULL");
+  if (n_rows < 0 || n_rows >= ((int64_t)1 << 31)) return fail(LSIM_E_ARG, "n_rows out of range");
+  Dev d;
+  int rc = make_dev(c, p, 1, d);
+  if (rc) return rc;
+  rc = check_segments(seg, d.recomb);
+  if (rc) return rc;
+  const int T = (int)(5.0 / d.ts) + 1;
+  if (T < 2 || T > kParamMaxSamples) return fail(LSIM_E_ARG, "t_sampling must give 2 .. 63 current samples in 5 us");
+  if (d.nout > kMaxOut) return fail(LSIM_E_ARG, "too many output ticks");
+  const int64_t per = mode == 0 ? (int64_t)(n_edges - 1) * (n_edges - 1) : 1;
+  const int64_t n_el = seg->n * per;
+  size_t need;
+  lsim_param_backward_query(c, n_el, n_rows, &need);
+  if (need > ws_bytes) return fail(LSIM_E_SPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  Carve cv(ws);
+  ParamAdj* adj = cv.take<ParamAdj>(n_el);
+  Pieces* pieces = cv.take<Pieces>(n_rows);
+  double* dz = cv.take<double>(n_rows);
+  double* partial = cv.take<double>(n_rows * kNGrad);
+  int64_t* n_rows_dev = cv.take<int64_t>(1);
+  fill<int64_t>(n_rows_dev, 1, seg->n > 0 ? n_rows : 0, st);
+  if (seg->n > 0 && n_rows > 0) {
+    if (!ex || !ey || !et0 || !ene || !esig || !pixel_ids || !pair_order || !row_offsets || !fee_tape || !d_adc)
+      return fail(LSIM_E_ARG, "NULL argument (the electrons, the rows, the readout tape and d_adc are required)");
+    const int K = (2 * d.nn + 1) * (2 * d.nn + 1);
+    hipLaunchKernelGGL(k_fee_adjoint, dim3((unsigned)((n_rows + kFeeRows - 1) / kFeeRows)), dim3(kFeeRows * kWave), 0, st, d,
+                       (const FeeTape*)fee_tape, d_adc, d_z, (const int64_t*)n_rows_dev, n_rows, pieces, dz);
+    hipLaunchKernelGGL(k_param_adj_electrons, dim3((unsigned)((seg->n + 255) / 256)), dim3(256), 0, st, d, make_segin(seg),
+                       mode, diffusion, normals, bin_edges, n_edges, adj);
+    ParamE e{const_cast<double*>(ex), const_cast<double*>(ey), const_cast<double*>(et0), const_cast<double*>(ene),
+             const_cast<double*>(esig), nullptr};
+    hipLaunchKernelGGL(k_param_rows_adj, dim3((unsigned)n_rows), dim3(kParamWaves * kWave), 0, st, d, e, adj, pixel_ids,
+                       pair_order, row_offsets, K, T, diffusion, pieces, d_wfs, partial);
+  }
+  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(kReduceThreads), 0, st, d, partial, seg->n > 0 ? n_rows : 0, dz,
+                     (const int64_t*)n_rows_dev, n_rows, d_params);
+  HIP_TRY(hipGetLastError());
+  return LSIM_OK;
+}
+
 float lsim_rbf_cull_radius(float sigma) {
   // exp(-d^2 / (2 sigma^2)) is exactly 0 in float32 once d^2 / (2 sigma^2) > 106; 1 % margin
   return 1.01f * sqrtf(106.f * 2.f * sigma * sigma);

@@ -12,8 +12,15 @@
 // order (a stable sort by row, on the host side), and one workgroup per row adds the currents of
 // its pairs into per-wave float64 rows in LDS (lane k = sample k), summed in a fixed order:
 // deterministic, no atomics.
+//
+// Adjoint (reverse mode, as jax.grad takes it through simulate_parametrized): the readout's
+// step-function cotangent of every row (k_fee_adjoint) plus an optional cotangent on the
+// waveforms; per pair, wave sums of the cotangent times the partial derivatives of the current
+// in the pair's charge, distances, arrival time and longitudinal spread; per electron, factors
+// that carry those to the 7 parameters (k_param_adj_electrons).  Integer decisions (pixel ids,
+// the whole-tick split of t0, the readout's decisions) are held at their forward values.
 #pragma once
-#include "device.h"
+#include "backward.h"
 
 namespace lsim {
 
@@ -171,6 +178,261 @@ __global__ __launch_bounds__(kParamWaves* kWave) void k_param_rows(Dev d, ParamE
     double v = 0.0;
     for (int w = 0; w < kParamWaves; ++w) v += acc[w][o];
     wfs[(int64_t)r * nout + o] = (float)v;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Adjoint
+// ------------------------------------------------------------------------------------------
+
+// Chain factors of one electron: the derivatives of its charge, position, arrival time and
+// longitudinal spread in the parameters that move them.
+struct __align__(16) ParamAdj {
+  double dne[5];   // d ne / d (Ab, kb, eField, lifetime, tran_diff)
+  double dx[2];    // d x / d (eField, tran_diff)
+  double dy[2];    // d y / d (eField, tran_diff)
+  double dt0[2];   // d t0 / d (eField, long_diff)
+  double dsig[2];  // d sigma / d (eField, long_diff)
+  double pad[3];
+};
+static_assert(sizeof(ParamAdj) == 128, "ParamAdj layout");
+
+// One thread per segment, in the electron order of k_param_electrons: the same physics (float64),
+// then the factors.  Charge: quench, lifetime and drift time as in the LUT chain; with transverse
+// bins times the bin weights w_i w_j, whose derivative in sigma_T is that of the erf at each inner
+// edge.  With normal draws the position moves by g sigma_T (and z by g sigma_L when the current
+// carries no diffusion).  t0 = |z - z_anode| / v and sigma = sigma_L / v move with v (eField).
+__global__ void k_param_adj_electrons(Dev d, SegIn in, int mode, int diffusion, const float* __restrict__ normals,
+                                      const double* __restrict__ edges, int n_edges, ParamAdj* __restrict__ adj) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= in.n) return;
+  Phys ph;
+  seg_physics(d, in, s, ph);
+  const double v = d.v, dv = d.dv;
+  const double att = exp(-ph.td / d.lifetime) * (double)ph.mask;
+  const double dne_E = ph.dq_dE_rec * att + ph.ne * ph.td * dv / (v * d.lifetime);
+  const double dst_E = -ph.st * dv / (2 * v), dst_T = d.tran_diff > 0 ? ph.st / (2 * d.tran_diff) : 0.0;
+  const double dsl_E = -ph.sl * dv / (2 * v), dsl_L = d.long_diff > 0 ? ph.sl / (2 * d.long_diff) : 0.0;
+  const int nb = mode == 0 ? n_edges - 1 : 1;
+  double w[kParamMaxEdges - 1], dw[kParamMaxEdges - 1];
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  if (mode == 0) {
+    double prev = -1.0, dprev = 0.0;
+    for (int k = 0; k < nb; ++k) {
+      double next = 1.0, dnext = 0.0;
+      if (k + 1 < nb) {
+        const double xi = edges[k + 1] / (M_SQRT2 * ph.st);
+        next = erf(xi);
+        dnext = M_2_SQRTPI * exp(-xi * xi) * (-xi / ph.st);
+      }
+      w[k] = 0.5 * (next - prev);
+      dw[k] = 0.5 * (dnext - dprev);
+      prev = next;
+      dprev = dnext;
+    }
+  } else {
+    gx = (double)normals[s * 3 + 0];
+    gy = (double)normals[s * 3 + 1];
+    if (!diffusion) gz = (double)normals[s * 3 + 2];
+    w[0] = 1.0;
+    dw[0] = 0.0;
+  }
+  const double z = (double)ldcol(in.z, s, in.stride) - d.shift[2] + gz * ph.sl;
+  const double z_an = d.tpc[ph.plane][2][0];
+  const double t0 = fabs(z - z_an) / v;
+  const double sz = z > z_an ? 1.0 : (z < z_an ? -1.0 : 0.0);
+  const double sig = ph.sl / v;
+  ParamAdj A;
+  A.dx[0] = gx * dst_E;
+  A.dx[1] = gx * dst_T;
+  A.dy[0] = gy * dst_E;
+  A.dy[1] = gy * dst_T;
+  A.dt0[0] = -t0 * dv / v + sz * gz * dsl_E / v;
+  A.dt0[1] = sz * gz * dsl_L / v;
+  A.dsig[0] = dsl_E / v - sig * dv / v;
+  A.dsig[1] = dsl_L / v;
+  A.pad[0] = A.pad[1] = A.pad[2] = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    for (int j = 0; j < nb; ++j) {
+      const double W = w[i] * w[j], dW = dw[i] * w[j] + w[i] * dw[j];
+      A.dne[0] = ph.dq_dAb * att * W;
+      A.dne[1] = ph.dq_dkb * att * W;
+      A.dne[2] = dne_E * W + ph.ne * dW * dst_E;
+      A.dne[3] = ph.ne * ph.td / (d.lifetime * d.lifetime) * W;
+      A.dne[4] = ph.ne * dW * dst_T;
+      adj[s * nb * nb + i * nb + j] = A;
+    }
+  }
+}
+
+// One tail of a pair's current at sample k (param_tail) and its partial derivatives in the
+// tail's location, decay constant and longitudinal spread.  With diffusion the tail is
+// (F_k - F_{k+1}) / (F_1 - F_{T-1}) / dt of H(g) = 1/2 erf(u) - P, u = (g - mu) / (sqrt2 sigma),
+// P = 1/2 exp(E) erfc(z), E = lambda (mu - g) + lambda^2 sigma^2 / 2,
+// z = (mu + lambda sigma^2 - g) / (sqrt2 sigma), lambda = 1 / scale; since E - z^2 = -u^2:
+//   dH/dmu = -lambda P,  dH/dsigma = lambda (phi(u) / (2 sqrt2) - lambda sigma P),
+//   dH/dlambda = phi(u) sigma / (2 sqrt2) - (lambda sigma^2 - (g - mu)) P,  phi(u) = 2/sqrt(pi) exp(-u^2),
+// and the quotient rule over the neighbouring lanes' values.  Without diffusion the tail is
+// (G_{k+1} - G_k + G_e / T) / dt with G = exp(min(0, (mu - g) / scale)).
+struct TailD {
+  double v, dloc, dscale, dsig;
+};
+
+__device__ __forceinline__ TailD param_tail_d(double gj, double loc, double scale, double sigma, double dt, int T,
+                                              bool diffusion) {
+  TailD r;
+  if (diffusion) {
+    const double lam = 1.0 / scale;
+    const double D = gj - loc;
+    const double u = D / (M_SQRT2 * sigma);
+    const double E = lam * (-D) + 0.5 * lam * lam * sigma * sigma;
+    const double z = (loc + lam * sigma * sigma - gj) / (M_SQRT2 * sigma);
+    const double P = 0.5 * exp(E) * erfc(z);
+    const double phi = M_2_SQRTPI * exp(-u * u);
+    const double F = 0.5 * erf(u) - P;
+    const double Fm = -lam * P;
+    const double Fs = lam * (phi * (0.5 * M_SQRT1_2) - lam * sigma * P);
+    const double Fc = -lam * lam * (phi * sigma * (0.5 * M_SQRT1_2) - (lam * sigma * sigma - D) * P);
+    const double Fn = __shfl_down(F, 1, kWave), Fmn = __shfl_down(Fm, 1, kWave);
+    const double Fsn = __shfl_down(Fs, 1, kWave), Fcn = __shfl_down(Fc, 1, kWave);
+    const double F1 = __shfl(F, 1, kWave), FT = __shfl(F, T - 1, kWave);
+    const double Fm1 = __shfl(Fm, 1, kWave), FmT = __shfl(Fm, T - 1, kWave);
+    const double Fs1 = __shfl(Fs, 1, kWave), FsT = __shfl(Fs, T - 1, kWave);
+    const double Fc1 = __shfl(Fc, 1, kWave), FcT = __shfl(Fc, T - 1, kWave);
+    const double den = F1 - FT;
+    const double q = (F - Fn) / den;
+    const double inv = 1.0 / (den * dt);
+    r.v = q / dt;
+    r.dloc = ((Fm - Fmn) - q * (Fm1 - FmT)) * inv;
+    r.dsig = ((Fs - Fsn) - q * (Fs1 - FsT)) * inv;
+    r.dscale = ((Fc - Fcn) - q * (Fc1 - FcT)) * inv;
+    return r;
+  }
+  const double y = (loc - gj) / scale;
+  const double G = exp(fmin(0.0, y));
+  const double Gl = y < 0 ? G / scale : 0.0;
+  const double Gc = y < 0 ? -G * y / scale : 0.0;
+  const double ye = (loc - dt / 2) / scale;
+  const double Ge = exp(fmin(0.0, ye));
+  const double Gel = ye < 0 ? Ge / scale : 0.0;
+  const double Gec = ye < 0 ? -Ge * ye / scale : 0.0;
+  const double Gn = __shfl_down(G, 1, kWave), Gln = __shfl_down(Gl, 1, kWave), Gcn = __shfl_down(Gc, 1, kWave);
+  r.v = (Gn - G + Ge / T) / dt;
+  r.dloc = (Gln - Gl + Gel / T) / dt;
+  r.dscale = (Gcn - Gc + Gec / T) / dt;
+  r.dsig = 0.0;
+  return r;
+}
+
+// One workgroup per row, over the same pairs as k_param_rows: the row's cotangent G(o) (the
+// readout's step function plus the optional waveform cotangent gw[r][o]) in LDS; per pair, lane k
+// takes G at the pair's tick o and the partials of sample k of the current; wave sums give
+// d loss / d (charge, x, y, t0, sigma) of the pair, and the electron's factors the 7 gradients.
+// Lanes with G(o) = 0 add nothing (a pair that overflows to NaN then adds no NaN where the loss
+// does not see it).  partial[r][7]: the row's sums, waves combined in index order.
+__global__ __launch_bounds__(kParamWaves* kWave) void k_param_rows_adj(Dev d, ParamE e, const ParamAdj* __restrict__ adj,
+                                                                        const int64_t* __restrict__ upix,
+                                                                        const int* __restrict__ order,
+                                                                        const int* __restrict__ offs, int K, int T,
+                                                                        int diffusion, const Pieces* __restrict__ pieces,
+                                                                        const float* __restrict__ gw,
+                                                                        double* __restrict__ partial) {
+  constexpr int NT = kParamWaves * kWave;
+  __shared__ double G[kMaxOut];
+  __shared__ double red[kNGrad][kParamWaves];
+  __shared__ int s_any;
+  const int r = blockIdx.x;
+  const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+  const int nout = d.nout;
+  if (t == 0) s_any = 0;
+  __syncthreads();
+  {
+    const Pieces* pr = pieces + r;
+    const int nf = pr->n;
+    bool any = false;
+    for (int o = t; o < nout; o += NT) {
+      double cf = 0.0;
+      for (int k = nf - 1; k >= 0; --k)
+        if (pr->x[k] >= o) cf = (double)pr->c[k];
+      const double v = cf + (gw ? (double)gw[(int64_t)r * nout + o] : 0.0);
+      G[o] = v;
+      any = any || v != 0.0;
+    }
+    if (any) s_any = 1;
+  }
+  __syncthreads();
+  double g[kNGrad];
+#pragma unroll
+  for (int k = 0; k < kNGrad; ++k) g[k] = 0.0;
+  const int64_t pid = upix[r];
+  if (s_any && pid >= 0) {
+    const int64_t nx = d.nx, ny = d.ny;
+    const int px = (int)(pid % nx), py = (int)((pid / nx) % ny), plane = (int)((pid / (nx * ny)) % d.ntpc);
+    const double cx = (double)px * d.pitch + d.tpc[plane][0][0] + d.pitch / 2;
+    const double cy = (double)py * d.pitch + d.tpc[plane][1][0] + d.pitch / 2;
+    const double dt = 5.0 / (double)(T - 1);
+    const int kk = lane < T ? lane : T - 1;
+    const double tk = kk == T - 1 ? 5.0 : (double)kk * dt;
+    const double gj = lane < T ? -tk + dt / 2 : -5.0 - dt / 2;
+    const int b0 = offs[r], b1 = offs[r + 1];
+    for (int i = b0 + wid; i < b1; i += kParamWaves) {
+      const int pr = order[i];
+      const int64_t el = pr / K;
+      const double t0 = e.t0[el];
+      const int t0_tick = (int)(t0 / d.ts + 0.5);
+      const int o = t0_tick - T + kk;
+      const double gv = (lane < T && o >= 0 && o < nout) ? G[o] : 0.0;
+      if (__ballot(gv != 0.0) == 0) continue;  // the same pair in every lane
+      const double ex = e.x[el] - cx, ey = e.y[el] - cy;
+      const double xd = fabs(ex), yd = fabs(ey);
+      const double ar = 1.060 - 0.909 * xd - 0.909 * yd + 5.856 * xd * yd + 0.207 * xd * xd + 0.207 * yd * yd;
+      const double b = 0.679 - 1.083 * xd - 1.083 * yd + 8.772 * xd * yd - 5.521 * xd * xd - 5.521 * yd * yd;
+      const double c = 2.644 - 9.174 * xd - 9.174 * yd + 13.483 * xd * yd + 45.887 * xd * xd + 45.887 * yd * yd;
+      const double ts0 = 2.948 - 2.705 * xd - 2.705 * yd + 4.825 * xd * yd + 20.814 * xd * xd + 20.814 * yd * yd;
+      const bool clamp = ar > 1.0;
+      const double a = clamp ? 1.0 : ar;
+      const double ax = clamp ? 0.0 : -0.909 + 5.856 * yd + 2 * 0.207 * xd;
+      const double ay = clamp ? 0.0 : -0.909 + 5.856 * xd + 2 * 0.207 * yd;
+      const double bx = -1.083 + 8.772 * yd - 2 * 5.521 * xd, by = -1.083 + 8.772 * xd - 2 * 5.521 * yd;
+      const double cxd = -9.174 + 13.483 * yd + 2 * 45.887 * xd, cyd = -9.174 + 13.483 * xd + 2 * 45.887 * yd;
+      const double tx = -2.705 + 4.825 * yd + 2 * 20.814 * xd, ty = -2.705 + 4.825 * xd + 2 * 20.814 * yd;
+      const double st0 = (t0 - t0_tick * d.ts) + ts0;
+      const double sig = e.sig[el];
+      const TailD tb = param_tail_d(gj, -st0, b, sig, dt, T, diffusion != 0);
+      const TailD tc = param_tail_d(gj, -st0, c, sig, dt, T, diffusion != 0);
+      const double cur = a * tb.v + (1 - a) * tc.v;
+      const double c_t0 = -(a * tb.dloc + (1 - a) * tc.dloc);
+      const double c_sig = a * tb.dsig + (1 - a) * tc.dsig;
+      const double c_x = ax * (tb.v - tc.v) + a * tb.dscale * bx + (1 - a) * tc.dscale * cxd + c_t0 * tx;
+      const double c_y = ay * (tb.v - tc.v) + a * tb.dscale * by + (1 - a) * tc.dscale * cyd + c_t0 * ty;
+      const double ne = e.ne[el];
+      const bool live = gv != 0.0;
+      const double q_ne = wave_sum(live ? gv * cur : 0.0);
+      const double q_x = wave_sum(live ? gv * ne * c_x : 0.0);
+      const double q_y = wave_sum(live ? gv * ne * c_y : 0.0);
+      const double q_t0 = wave_sum(live ? gv * ne * c_t0 : 0.0);
+      const double q_sig = wave_sum(live ? gv * ne * c_sig : 0.0);
+      // d |x - cx| / d x = sign(x - cx) (0 at 0)
+      const double gx = ex > 0 ? q_x : (ex < 0 ? -q_x : 0.0);
+      const double gy = ey > 0 ? q_y : (ey < 0 ? -q_y : 0.0);
+      const ParamAdj* A = adj + el;
+      g[0] += q_ne * A->dne[0];
+      g[1] += q_ne * A->dne[1];
+      g[2] += q_ne * A->dne[2] + gx * A->dx[0] + gy * A->dy[0] + q_t0 * A->dt0[0] + q_sig * A->dsig[0];
+      g[3] += q_ne * A->dne[3];
+      g[5] += q_t0 * A->dt0[1] + q_sig * A->dsig[1];
+      g[6] += q_ne * A->dne[4] + gx * A->dx[1] + gy * A->dy[1];
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kNGrad; ++k) red[k][wid] = g[k];
+  }
+  __syncthreads();
+  if (t < kNGrad) {
+    double v = 0.0;
+    for (int w = 0; w < kParamWaves; ++w) v += red[t][w];
+    partial[(int64_t)r * kNGrad + t] = v;
   }
 }
 

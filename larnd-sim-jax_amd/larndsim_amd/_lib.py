@@ -24,7 +24,7 @@ EXPORTS = (
     "lsim_segment_records", "lsim_timing", "lsim_timing_read", "lsim_rbf_sums", "lsim_rbf_sums_grid",
     "lsim_rbf_cull_radius", "lsim_nearest", "lsim_chop_query", "lsim_chop_count", "lsim_chop_fill",
     "lsim_prob_fee_query", "lsim_prob_fee", "lsim_prob_fee_backward", "lsim_sdtw_query", "lsim_sdtw",
-    "lsim_param_electrons", "lsim_param_rows",
+    "lsim_param_electrons", "lsim_param_rows", "lsim_param_backward_query", "lsim_param_backward",
 )
 
 
@@ -127,6 +127,9 @@ def load(path: str = LIB_PATH):
         "lsim_param_electrons": [vp, C.POINTER(ParamsC), C.POINTER(Segments), i32, i32, vp, vp, i32, vp, vp, vp, vp,
                                  vp, vp, vp, vp],
         "lsim_param_rows": [vp, C.POINTER(ParamsC), vp, vp, vp, vp, vp, vp, i64, vp, vp, i32, vp, vp],
+        "lsim_param_backward_query": [vp, i64, i64, C.POINTER(sz)],
+        "lsim_param_backward": [vp, C.POINTER(ParamsC), C.POINTER(Segments), i32, i32, vp, vp, i32, vp, vp, vp, vp, vp,
+                                vp, i64, vp, vp, vp, vp, vp, vp, vp, sz, vp, vp],
         "lsim_sdtw_query": [i64, i64, C.POINTER(sz)],
         "lsim_sdtw": [vp, i64, vp, i64, C.c_float, vp, sz, vp, vp, vp],
         "lsim_chop_query": [i64, C.POINTER(sz)],

@@ -315,8 +315,9 @@ class Simulator:
                                                ids.numel(), C.c_void_p(n.data_ptr()), self._stream()))
         return ids, n
 
-    def fee(self, params, wfs, pixel_ids):
-        """Self-trigger + digitisation + compaction of caller waveforms (simulate_stochastic)."""
+    def fee(self, params, wfs, pixel_ids, want_tape=False):
+        """Self-trigger + digitisation + compaction of caller waveforms (simulate_stochastic); with
+        ``want_tape`` the decisions the adjoint replays are kept in ``out["tape"]``."""
         _check_noise(params)
         wfs = wfs.to(device=self.device, dtype=torch.float32).contiguous()
         pids = pixel_ids.to(device=self.device, dtype=torch.int64).contiguous()
@@ -326,7 +327,7 @@ class Simulator:
         sz = C.c_size_t()
         _lib.check(self.lib.lsim_fee_query(self._ctx, P, C.byref(sz)))
         ws = torch.empty(int(sz.value), dtype=torch.uint8, device=self.device)
-        out = self.alloc_out(P, want_wfs=False)
+        out = self.alloc_out(P, want_wfs=False, want_tape=want_tape)
         oc = self.out_struct(out)
         _lib.check(self.lib.lsim_fee_forward(self._ctx, C.byref(params_struct(params)), C.c_void_p(wfs.data_ptr()),
                                              C.c_void_p(pids.data_ptr()), P, C.c_void_p(ws.data_ptr()), ws.numel(),

@@ -105,7 +105,8 @@ class HipLUTProbabilisticSimulation(SimulationStrategy):
 class HipParametrizedSimulation(SimulationStrategy):
     """The parametrized current model on the GPU (strategies.py:136-149): the compacted hit arrays
     of ``param.simulate_parametrized`` under the reference's keys.  ``rngkey`` seeds the normal
-    draws of mc_diff.  Forward only."""
+    draws of mc_diff.  With parameters that require gradients, ``adcs`` and ``pixel_z`` carry
+    autograd history through the model's adjoint."""
 
     def predict(self, params, tracks, fields, rngkey=0):
         from .param import simulate_parametrized

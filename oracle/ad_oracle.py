@@ -34,6 +34,34 @@ def vdrift_t(E, p):
     return num / den * (p.temperature / 89) ** -1.5 / 1000 * E
 
 
+def fee_replay(p, dec, ticks, U, wfs_out, v):
+    """The self-trigger with the oracle's decisions (fee_jax.py:189-268), digitize (fee_jax.py:68-69)
+    and hit z (detsim_jax.py:308-319) as torch float64 functions of the waveforms wfs_out (P, T)
+    and the drift velocity v: (adc grid (P, H), z grid (P, H))."""
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64)  # noqa: E731
+    P = wfs_out.shape[0]
+    H = int(p.MAX_ADC_VALUES)
+    Q = torch.cumsum(wfs_out * p.t_sampling, dim=1)
+    rows = torch.arange(P)
+    adcs = []
+    for k in range(H):
+        e = torch.as_tensor(dec["e"][:, k])
+        g = torch.as_tensor(dec["g"][:, k])
+        valid = torch.as_tensor(dec["valid"][:, k])
+        a = torch.where(valid, Q[rows, e], torch.zeros((), dtype=torch.float64))
+        adcs.append(a)
+        Qn = Q - Q[rows, g][:, None]
+        Q = torch.where(Qn < 0, torch.zeros((), dtype=torch.float64), Qn)
+    integral = torch.stack(adcs, dim=1)
+    x = torch.clamp(integral * p.GAIN + p.V_PEDESTAL - p.V_CM, min=0)
+    adc = torch.clamp(x * p.ADC_COUNTS / (p.V_REF - p.V_CM), max=p.ADC_COUNTS)
+    bb = np.asarray(p.tpc_borders, np.float64)
+    _, _, plane, _ = O.id2pixel(p, U)
+    za, zc = t(bb[plane, 2, 0]), t(bb[plane, 2, 1])
+    z = za[:, None] + t(ticks) * p.t_sampling * v * torch.sign(zc - za)[:, None]
+    return adc, z
+
+
 class Frozen:
     """Integer decisions and parameter-independent data of one event, from the NumPy oracle."""
 
@@ -171,26 +199,7 @@ class Frozen:
         acc(sg, nb["row"], win, delta)
 
         wfs_out = wfs.reshape(P, nticks)[:, 1:]
-        # self-trigger replay with the oracle's decisions
-        H = int(p.MAX_ADC_VALUES)
-        Q = torch.cumsum(wfs_out * p.t_sampling, dim=1)
-        rows = torch.arange(P)
-        adcs = []
-        for k in range(H):
-            e = torch.as_tensor(self.dec["e"][:, k])
-            g = torch.as_tensor(self.dec["g"][:, k])
-            valid = torch.as_tensor(self.dec["valid"][:, k])
-            a = torch.where(valid, Q[rows, e], torch.zeros((), dtype=torch.float64))
-            adcs.append(a)
-            Qn = Q - Q[rows, g][:, None]
-            Q = torch.where(Qn < 0, torch.zeros((), dtype=torch.float64), Qn)
-        integral = torch.stack(adcs, dim=1)
-        x = torch.clamp(integral * p.GAIN + p.V_PEDESTAL - p.V_CM, min=0)
-        adc = torch.clamp(x * p.ADC_COUNTS / (p.V_REF - p.V_CM), max=p.ADC_COUNTS)
-        bb = np.asarray(p.tpc_borders, np.float64)
-        _, _, plane, _ = O.id2pixel(p, self.U)
-        za, zc = t(bb[plane, 2, 0]), t(bb[plane, 2, 1])
-        z = za[:, None] + t(self.ticks) * p.t_sampling * v * torch.sign(zc - za)[:, None]
+        adc, z = fee_replay(p, self.dec, self.ticks, self.U, wfs_out, v)
         return adc, z, wfs_out
 
     def theta0(self):

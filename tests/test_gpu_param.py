@@ -90,6 +90,7 @@ def test_strategy_and_reference_outputs(params4, c1):
     e7 = GP.electrons(p, trd, f, rngseed=7)
     e8 = GP.electrons(p, trd, f, rngseed=8)
     assert not torch.equal(e7["x"], e8["x"])
-    with pytest.raises(NotImplementedError):
-        HipParametrizedSimulation().predict(p.replace(Ab=torch.tensor(0.8, dtype=torch.float64, requires_grad=True)),
-                                            trd, f)
+    # parameters that require gradients: the same hits, now with autograd history
+    g = HipParametrizedSimulation().predict(p.replace(Ab=torch.tensor(0.8, dtype=torch.float64, requires_grad=True)),
+                                            trd, f, rngkey=7)
+    assert g["adcs"].grad_fn is not None and torch.equal(g["adcs"].detach(), out["adcs"])

@@ -259,6 +259,13 @@ def test_new_entry_points_check_arguments(lib):
                                     None) < 0
     assert lib.lsim_param_rows(None, None, None, None, None, None, None, None, 0, None, None, 1, None, None) < 0
     assert lib.lsim_param_rows(pp, None, None, None, None, None, None, None, 0, None, None, 1, None, None) < 0
+    # its adjoint: sizes and NULLs
+    assert lib.lsim_param_backward_query(None, 1, 1, C.byref(sz)) < 0
+    assert lib.lsim_param_backward_query(pp, -1, 1, C.byref(sz)) < 0
+    assert lib.lsim_param_backward_query(pp, 10, 4, C.byref(sz)) == 0 and sz.value >= 10 * 128 + 4 * 7 * 8
+    assert lib.lsim_param_backward(pp, None, None, 1, 1, None, None, 0, None, None, None, None, None, None, 0, None,
+                                   None, None, None, None, None, None, 0, None, None) < 0
+    assert b"NULL" in lib.lsim_last_error()
     # scheduling knobs: out-of-range values are refused on the host
     assert lib.lsim_set_row_tile(0) < 0 and lib.lsim_set_row_tile(4097) < 0 and b"tile" in lib.lsim_last_error()
     assert lib.lsim_set_row_tile(16) == 0
