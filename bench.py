@@ -478,13 +478,42 @@ def prob_fee_leg(params, wfs, steps, cpu=True, cpu_rows=16):
     return leg
 
 
+def _param_fwd_bwd(p, trk, fields, steps, ms_fwd):
+    """Forward + backward (cotangents on every hit's ADC value and z) through the parametrized
+    model's adjoint, against the 15 forwards (1 + 2 x 7) of a central-difference gradient."""
+    import torch
+
+    from larndsim_amd import param as GP
+    from larndsim_amd.autograd import theta_of
+
+    def fwd_bwd():
+        theta = theta_of(p).cuda().requires_grad_(True)
+        adcs, z, _ = GP.ParamSimFunction.apply(theta, p, trk, fields, 1, None, None)
+        (adcs.double().sum() + z.double().sum()).backward()
+        return theta.grad
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        grad = fwd_bwd()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fb = e0.elapsed_time(e1) / steps
+    return dict(ms_fwd_bwd=ms_fb, fd_gradient_ms=15 * ms_fwd, adjoint_vs_fd=15 * ms_fwd / ms_fb,
+                grad_finite=bool(torch.isfinite(grad).all()))
+
+
 def param_leg(params, trk, fields, steps, cpu=True, cpu_segments=10000):
     """The parametrized current model (simulate_parametrized, sim_jax.py:339-372; mc_diff, one
     electron per segment, diffusion in the current, 4 neighbour pixels) on the 200k-segment
     event, inputs resident: the whole call (electrons, pixel set, waveforms, readout, hits) by
     events on torch's current stream, the waveform kernel alone, and forward + backward through
-    the adjoint against the 15 forwards of a central-difference gradient.  CPU baseline: the
-    float64 NumPy oracle on the first cpu_segments segments, 1 thread."""
+    the adjoint against the 15 forwards of a central-difference gradient; the same without
+    diffusion in the current.  Rows that meet an overflowing pair are NaN, as in the reference
+    (``rows_nan``), and carry no hits.  CPU baseline: the float64 NumPy oracle on the first
+    cpu_segments segments, 1 thread."""
     import numpy as np
     import torch
 
@@ -511,29 +540,24 @@ def param_leg(params, trk, fields, steps, cpu=True, cpu_segments=10000):
     ms_w = e0.elapsed_time(e1) / steps
     n = int(trk.shape[0])
     K = (2 * int(p.number_pix_neighbors) + 1) ** 2
-    # forward + backward (hit ADC and z cotangents) through the model's adjoint, against the
-    # finite-difference gradient's 15 forwards (1 + 2 x 7)
-    from larndsim_amd.autograd import theta_of
-
-    def fwd_bwd():
-        theta = theta_of(p).cuda().requires_grad_(True)
-        adcs, z, _ = GP.ParamSimFunction.apply(theta, p, trk, fields, 1, None, None)
-        (adcs.double().sum() + z.double().sum()).backward()
-        return theta.grad
-
-    fwd_bwd()
+    rows_nan = int(torch.isnan(wfs).any(dim=1).sum())
+    leg = dict(workload=f"200k-segment event, mc_diff, {K} pixels per electron, 51 current samples", segments=n,
+               pairs=n * K, rows=int(U.numel()), rows_nan=rows_nan, hits=int(out[0].numel()), ms=ms,
+               segments_per_s=n / (ms * 1e-3), waveforms_ms=ms_w, pairs_per_s=n * K / (ms_w * 1e-3))
+    leg.update(_param_fwd_bwd(p, trk, fields, steps, ms))
+    # without diffusion in the current (one-sided exponentials, finite everywhere): every row
+    # carries hits, so the adjoint visits most pairs
+    pn = p.replace(diffusion_in_current_sim=False)
+    GP.simulate_parametrized(pn, trk, fields, rngseed=1)
     torch.cuda.synchronize()
     e0.record()
     for _ in range(steps):
-        grad = fwd_bwd()
+        outn = GP.simulate_parametrized(pn, trk, fields, rngseed=1)
     e1.record()
     torch.cuda.synchronize()
-    ms_fb = e0.elapsed_time(e1) / steps
-    leg = dict(workload=f"200k-segment event, mc_diff, {K} pixels per electron, 51 current samples", segments=n,
-               pairs=n * K, rows=int(U.numel()), hits=int(out[0].numel()), ms=ms, segments_per_s=n / (ms * 1e-3),
-               waveforms_ms=ms_w, pairs_per_s=n * K / (ms_w * 1e-3), ms_fwd_bwd=ms_fb,
-               fd_gradient_ms=15 * ms, adjoint_vs_fd=15 * ms / ms_fb,
-               grad_finite=bool(torch.isfinite(grad).all()))
+    ms_n = e0.elapsed_time(e1) / steps
+    leg["no_diffusion"] = dict(ms=ms_n, segments_per_s=n / (ms_n * 1e-3), hits=int(outn[0].numel()),
+                               **_param_fwd_bwd(pn, trk, fields, steps, ms_n))
     if cpu:
         sys.path.insert(0, ROOT)
         from threadpoolctl import threadpool_limits

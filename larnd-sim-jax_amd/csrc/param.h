@@ -272,8 +272,11 @@ __global__ void k_param_adj_electrons(Dev d, SegIn in, int mode, int diffusion, 
 // z = (mu + lambda sigma^2 - g) / (sqrt2 sigma), lambda = 1 / scale; since E - z^2 = -u^2:
 //   dH/dmu = -lambda P,  dH/dsigma = lambda (phi(u) / (2 sqrt2) - lambda sigma P),
 //   dH/dlambda = phi(u) sigma / (2 sqrt2) - (lambda sigma^2 - (g - mu)) P,  phi(u) = 2/sqrt(pi) exp(-u^2),
-// and the quotient rule over the neighbouring lanes' values.  Without diffusion the tail is
-// (G_{k+1} - G_k + G_e / T) / dt with G = exp(min(0, (mu - g) / scale)).
+// and the quotient rule over the neighbouring lanes' values.  Every lane divides its H, P and phi
+// by the shared denominator F_1 - F_{T-1} before the partials are formed: near the float64 limit
+// (P ~ 1e306 for a decay constant near zero) the tail is finite while lambda P alone would not be;
+// past it (an infinite denominator) the samples are 0 and their partials are taken as 0.
+// Without diffusion the tail is (G_{k+1} - G_k + G_e / T) / dt with G = exp(min(0, (mu - g) / scale)).
 struct TailD {
   double v, dloc, dscale, dsig;
 };
@@ -288,24 +291,29 @@ __device__ __forceinline__ TailD param_tail_d(double gj, double loc, double scal
     const double E = lam * (-D) + 0.5 * lam * lam * sigma * sigma;
     const double z = (loc + lam * sigma * sigma - gj) / (M_SQRT2 * sigma);
     const double P = 0.5 * exp(E) * erfc(z);
-    const double phi = M_2_SQRTPI * exp(-u * u);
     const double F = 0.5 * erf(u) - P;
-    const double Fm = -lam * P;
-    const double Fs = lam * (phi * (0.5 * M_SQRT1_2) - lam * sigma * P);
-    const double Fc = -lam * lam * (phi * sigma * (0.5 * M_SQRT1_2) - (lam * sigma * sigma - D) * P);
-    const double Fn = __shfl_down(F, 1, kWave), Fmn = __shfl_down(Fm, 1, kWave);
+    const double den = __shfl(F, 1, kWave) - __shfl(F, T - 1, kWave);
+    const double p = P / den, f = F / den;
+    const double ph = M_2_SQRTPI * exp(-u * u) * (0.5 * M_SQRT1_2) / den;  // phi(u) / (2 sqrt2) / den
+    const double Fm = -lam * p;
+    const double Fs = lam * (ph - lam * sigma * p);
+    const double Fc = -lam * (lam * (ph * sigma - (lam * sigma * sigma - D) * p));
+    const double fn = __shfl_down(f, 1, kWave), Fmn = __shfl_down(Fm, 1, kWave);
     const double Fsn = __shfl_down(Fs, 1, kWave), Fcn = __shfl_down(Fc, 1, kWave);
-    const double F1 = __shfl(F, 1, kWave), FT = __shfl(F, T - 1, kWave);
     const double Fm1 = __shfl(Fm, 1, kWave), FmT = __shfl(Fm, T - 1, kWave);
     const double Fs1 = __shfl(Fs, 1, kWave), FsT = __shfl(Fs, T - 1, kWave);
     const double Fc1 = __shfl(Fc, 1, kWave), FcT = __shfl(Fc, T - 1, kWave);
-    const double den = F1 - FT;
-    const double q = (F - Fn) / den;
-    const double inv = 1.0 / (den * dt);
+    const double q = f - fn;
     r.v = q / dt;
-    r.dloc = ((Fm - Fmn) - q * (Fm1 - FmT)) * inv;
-    r.dsig = ((Fs - Fsn) - q * (Fs1 - FsT)) * inv;
-    r.dscale = ((Fc - Fcn) - q * (Fc1 - FcT)) * inv;
+    if (!isfinite(den)) {
+      // |den| beyond the float64 range: the samples are 0 (or NaN, and the row with them), and so
+      // are their derivatives in the limit; the overflowing lanes would give NaN here
+      r.dloc = r.dsig = r.dscale = 0.0;
+      return r;
+    }
+    r.dloc = ((Fm - Fmn) - q * (Fm1 - FmT)) / dt;
+    r.dsig = ((Fs - Fsn) - q * (Fs1 - FsT)) / dt;
+    r.dscale = ((Fc - Fcn) - q * (Fc1 - FcT)) / dt;
     return r;
   }
   const double y = (loc - gj) / scale;

@@ -96,3 +96,45 @@ def test_param_strategy_gradients_and_no_grad(params4, c1):
         plain = s.predict(pg, torch.from_numpy(tr).cuda(), f, rngkey=3)
     assert plain["adcs"].grad_fn is None
     assert torch.equal(plain["adcs"], out["adcs"].detach()) and torch.equal(plain["ticks"], out["ticks"])
+
+
+@pytest.mark.parametrize("diffusion", [False, True], ids=["nodiff", "diff"])
+def test_param_adjoint_on_the_200k_segment_event(params4, diffusion):
+    """At the benchmark's size (200k segments, 81 pixels per electron, 16.2 M pairs) against central
+    differences of the GPU forward itself: a random cotangent on every tick of the finite rows, and
+    the parameters that move only the charge (Ab, kb, lifetime; no pixel or whole-tick decision
+    moves with them, the waveform is linear in Ab), to 1e-3."""
+    from larndsim_amd import synthetic
+    from larndsim_amd.autograd import theta_of
+    from larndsim_amd.param import ParamSimFunction
+
+    tr, f = synthetic.workload("c2")
+    trk = torch.from_numpy(np.ascontiguousarray(tr)).cuda()
+    p = params4.replace(mc_diff=True, diffusion_in_current_sim=diffusion)
+    print(f"[param-adj] 200k segments diffusion={diffusion}", flush=True)
+    theta = theta_of(p).cuda().requires_grad_(True)
+    _, _, wfs = ParamSimFunction.apply(theta, p, trk, f, 1, None, None)
+    finite = torch.isfinite(wfs).all(dim=1)
+    print(f"[param-adj] {wfs.shape[0]} rows, {int((~finite).sum())} with an overflowing pair", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    w = torch.randn(wfs.shape, generator=g, device="cuda", dtype=torch.float64) * finite[:, None]
+    (w * torch.nan_to_num(wfs.double(), nan=0.0)).sum().backward()
+    grad = theta.grad.cpu().numpy()
+    assert np.isfinite(grad).all()
+
+    def value(th):
+        with torch.no_grad():
+            _, _, wv = ParamSimFunction.apply(th, p, trk, f, 1, None, None)
+            assert torch.equal(torch.isfinite(wv).all(dim=1), finite)
+            return float((w * torch.nan_to_num(wv.double(), nan=0.0)).sum())
+
+    th0 = theta.detach()
+    for i, n in ((0, "Ab"), (1, "kb"), (3, "lifetime")):
+        # the float32 waveforms' rounding sets the differences' noise: a wide interval (the charge
+        # is linear in Ab and smooth in kb and lifetime: the truncation error stays below 1e-5)
+        h = 1e-2 * abs(float(th0[i]))
+        e = torch.zeros_like(th0)
+        e[i] = h
+        fd = (value(th0 + e) - value(th0 - e)) / (2 * h)
+        print(f"[param-adj] {n}: adjoint {grad[i]:.6e} differences {fd:.6e}", flush=True)
+        assert grad[i] == pytest.approx(fd, rel=1e-3), n
