@@ -234,13 +234,15 @@ def _counter_rows(d, kernel_sub):
     import csv
     import glob
 
-    vals = {}
+    vals, names = {}, {}
     for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
         with open(path) as fh:
             for row in csv.DictReader(fh):
                 if kernel_sub in row["Kernel_Name"]:
                     k = int(row["Dispatch_Id"])
                     vals[k] = vals.get(k, 0.0) + float(row["Counter_Value"])
+                    names[k] = row["Kernel_Name"].split("(")[0].replace("void ", "").replace("lsim::", "")
+    _counter_rows.names = [names[k] for k in sorted(vals)]
     return [vals[k] for k in sorted(vals)]
 
 
@@ -249,6 +251,7 @@ def record_traffic(d, kernel_ms):
     the 128-event batch; d/copy_fetch, d/copy_write for `--copy-calibrate`) into HBM bytes per
     launch, stamped with the kernel sources and the kernel time, into TRAFFIC_JSON."""
     fetch = _counter_rows(os.path.join(d, "fetch"), "k_rows_fwd")[1:]  # the first, cold dispatch excluded
+    kernels = sorted(set(_counter_rows.names[1:]))
     write = _counter_rows(os.path.join(d, "write"), "k_rows_fwd")[1:]
     cf = _counter_rows(os.path.join(d, "copy_fetch"), "copyBuffer")
     cw = _counter_rows(os.path.join(d, "copy_write"), "copyBuffer")
@@ -256,7 +259,7 @@ def record_traffic(d, kernel_ms):
     ff = sum(cf) / len(cf) * 1024 / copy_bytes
     wf = sum(cw) / len(cw) * 1024 / copy_bytes
     fk, wk = sum(fetch) / len(fetch), sum(write) / len(write)
-    rec = dict(kernel="k_rows_fwd<5,4>", workload="128 events, 9 160 058 segments, 457 979 pixel rows",
+    rec = dict(kernel=", ".join(kernels), workload="128 events, 9 160 058 segments, 457 979 pixel rows",
                dispatches_used=f"{len(fetch)} warm dispatches of `bench.py --profile-batch` (the first excluded)",
                fetch_size_kib=fk, write_size_kib=wk, copy_fetch_kib=sum(cf) / len(cf),
                copy_write_kib=sum(cw) / len(cw), copy_bytes=copy_bytes, fetch_factor=ff, write_factor=wf,
