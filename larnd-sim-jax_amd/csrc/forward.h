@@ -97,31 +97,52 @@ __global__ void k_cell_keys(Dev d, SegIn in, const uint32_t* __restrict__ vals, 
 // segment's entries lie in the tick-k0 plane; the waveform kernels then read dtot[ab][cell range]
 // contiguously.
 //
-// The table loads are issued in batches (kCorrBatch stripe entries, then one x-bin row of
-// diffusion bins at a time) so that each thread has several in flight; each entry still receives
-// its terms in the same order (its stripe term, then its diffusion bins by x-bin, y-bin).
+// The stripe entries come from DNS, the raw response's D of every (tick, sub-bin x, sub-bin y)
+// regrouped so that one segment's WW entries are contiguous (k_lut_dns).  A wave stages each batch
+// of kCorrBatch entries of its 64 segments through LDS: lane e of load m fetches element
+// e + 64 m of the batch (segment (e + 64 m) / kCorrBatch, entry (e + 64 m) % kCorrBatch), so one
+// load instruction reads about seven segments' runs of kCorrBatch contiguous entries instead of 64
+// scattered ones; each lane then takes its own segment's entries from LDS.  The diffusion-bin loads
+// are issued one x-bin row at a time so that each thread has several in flight; each entry still
+// receives its terms in the same order (its stripe term, then its diffusion bins by x-bin, y-bin).
 constexpr int kCorrBatch = 9;
+constexpr int kRecThreads = 256;
 
 template <int NB>
-__device__ __forceinline__ void seg_corrections(const Dev& d, const float2* __restrict__ DN, const float2* __restrict__ DM,
-                                                const SegRec& R, int ux, int uy, int64_t i, int64_t n,
+__device__ __forceinline__ void seg_corrections(const Dev& d, const float2* __restrict__ DNS, const float2* __restrict__ DM,
+                                                const SegRec& R, bool live, int ux, int uy, int64_t i, int64_t n,
+                                                float2* __restrict__ stage, int* __restrict__ base,
                                                 float* __restrict__ dtot) {
   constexpr int SYM = (NB - 1) / 2;
   const int W = 2 * d.ne + 1, WW = W * W, nsb = d.nsb, cm = d.cm;
+  const int lane = threadIdx.x & (kWave - 1);
   const float f = R.f;
-  const float2* DNk = DN + (int64_t)R.k0 * d.lx * d.ly;
-  const float2* DMk = DM + ((int64_t)R.k0 * d.nm + R.idx) * cm * cm;
+  // the WW entries of (k0, ux, uy) start here (fits an int: nt nsb^2 WW < 2^31 for every valid LUT)
+  base[lane] = live ? (int)((((int64_t)R.k0 * nsb + ux) * nsb + uy) * WW) : -1;
+  __builtin_amdgcn_wave_barrier();
   float dn[9];  // entries (a, b) in [-1, 1]^2, index (a + 1) * 3 + (b + 1)
   // stripe terms of every entry
   for (int ab0 = 0; ab0 < WW; ab0 += kCorrBatch) {
+#pragma unroll
+    for (int m = 0; m < kCorrBatch; ++m) {
+      const int e = lane + kWave * m;
+      const int sg = e / kCorrBatch, u = e - sg * kCorrBatch;
+      const int b = base[sg];
+      stage[e] = (b >= 0 && ab0 + u < WW) ? DNS[b + ab0 + u] : make_float2(0.f, 0.f);
+    }
+    // wave-local LDS: the stores and loads of one wave complete in program order; the wave
+    // barriers keep the compiler from moving them across
+    __builtin_amdgcn_wave_barrier();
     float2 D[kCorrBatch];
 #pragma unroll
     for (int u = 0; u < kCorrBatch; ++u) {
       const int ab = ab0 + u;
       const int a = ab / W - d.ne, b = ab % W - d.ne;
       const bool on = ab < WW && (a != 0 || b != 0) && a >= -d.nn && a <= d.nn && b >= -d.nn && b <= d.nn;
-      D[u] = on ? DNk[offset_bin(ux, a, nsb) * d.ly + offset_bin(uy, b, nsb)] : make_float2(0.f, 0.f);
+      D[u] = on ? stage[lane * kCorrBatch + u] : make_float2(0.f, 0.f);
     }
+    __builtin_amdgcn_wave_barrier();  // every lane has its entries before the next batch is staged
+    if (!live) continue;
     float dl[kCorrBatch];  // all of the batch's values before its stores: no store between two uses
 #pragma unroll
     for (int u = 0; u < kCorrBatch; ++u) {
@@ -145,6 +166,8 @@ __device__ __forceinline__ void seg_corrections(const Dev& d, const float2* __re
       }
     }
   }
+  if (!live) return;
+  const float2* DMk = DM + ((int64_t)R.k0 * d.nm + R.idx) * cm * cm;
   // diffusion bins of the idx template, one x-bin row of loads at a time
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
@@ -179,19 +202,14 @@ __device__ __forceinline__ void seg_corrections(const Dev& d, const float2* __re
   }
 }
 
-// One thread per segment in sorted order: the records and the boundary corrections.
-template <int NB>
-__global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
-                          const float2* __restrict__ DN, const float2* __restrict__ DM, SegRec* __restrict__ rec,
-                          PairRec* __restrict__ prs, float* __restrict__ dtot) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || keys[i] >= (uint32_t)d.n_cells) return;
+// The record of the segment with input index src at sorted position i, and its pair record.
+__device__ __forceinline__ void record(const Dev& d, const SegIn& in, int64_t src, int64_t i, SegRec& r, int& ux, int& uy,
+                                       SegRec* __restrict__ rec, PairRec* __restrict__ prs) {
   Phys ph;
-  seg_physics(d, in, vals[i], ph);
+  seg_physics(d, in, src, ph);
   double wx[kMaxNb], wy[kMaxNb];
   tran_weights(d, ph.x0, ph.st, wx, nullptr);
   tran_weights(d, ph.y0, ph.st, wy, nullptr);
-  SegRec r;
   r.bx = ph.bx;
   r.by = ph.by;
   r.k0 = ph.k0;
@@ -206,7 +224,8 @@ __global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, co
     r.wx[k] = k < d.nb ? (float)wx[k] : 0.f;
     r.wy[k] = k < d.nb ? (float)wy[k] : 0.f;
   }
-  const int ux = pymod_i(ph.bx, d.nsb), uy = pymod_i(ph.by, d.nsb);
+  ux = pymod_i(ph.bx, d.nsb);
+  uy = pymod_i(ph.by, d.nsb);
   r.uxy = ux | (uy << 16);
   rec[i] = r;
   PairRec P;
@@ -219,7 +238,25 @@ __global__ void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys, co
   P.U0 = offset_bin(ux, 0, d.nsb) * d.ly * kWinPad * 4;
   P.V0 = offset_bin(uy, 0, d.nsb) * kWinPad * 4;
   prs[i] = P;
-  seg_corrections<NB>(d, DN, DM, r, ux, uy, i, n, dtot);
+}
+
+// One thread per segment in sorted order (kRecThreads per workgroup): the records and the boundary
+// corrections.  Threads past the end or of dropped segments stay to stage their wave's loads.
+template <int NB>
+__global__ __launch_bounds__(kRecThreads) void k_records(Dev d, SegIn in, const uint32_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ vals, int64_t n,
+                                                         const float2* __restrict__ DNS, const float2* __restrict__ DM,
+                                                         SegRec* __restrict__ rec, PairRec* __restrict__ prs,
+                                                         float* __restrict__ dtot) {
+  __shared__ float2 stage[kRecThreads / kWave][kWave * kCorrBatch];
+  __shared__ int sbase[kRecThreads / kWave][kWave];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int wid = threadIdx.x / kWave;
+  const bool live = i < n && keys[i] < (uint32_t)d.n_cells;
+  SegRec r = {};
+  int ux = 0, uy = 0;
+  if (live) record(d, in, vals[i], i, r, ux, uy, rec, prs);
+  seg_corrections<NB>(d, DNS, DM, r, live, ux, uy, i, n, stage[wid], sbase[wid], dtot);
 }
 
 // Cell ranges [begin, end) in the sorted order and the "first segment of an on-plane cell" flag
@@ -1376,6 +1413,22 @@ __global__ __launch_bounds__(256) void k_lut_tables(Dev d, const float* __restri
 }
 
 // Padded (T_{m-1}, T_m, T_{m+1}) windows for m in [1, nm-2]; other slabs are zero.
+// DNS[k][ux][uy][ab] = DN[k][offset_bin(ux, a)][offset_bin(uy, b)] for the (2 ne + 1)^2 row offsets
+// (a, b) = (ab / W - ne, ab % W - ne) of a segment in sub-bin (ux, uy): the raw response's D entries
+// of one segment's neighbour stripes, contiguous.
+__global__ void k_lut_dns(Dev d, const float2* __restrict__ DN, float2* __restrict__ DNS) {
+  const int W = 2 * d.ne + 1, WW = W * W, nsb = d.nsb;
+  const int64_t total = (int64_t)d.nt * nsb * nsb * WW;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
+    const int ab = (int)(j % WW);
+    const int64_t q = j / WW;
+    const int uy = (int)(q % nsb), ux = (int)((q / nsb) % nsb);
+    const int64_t k = q / ((int64_t)nsb * nsb);
+    const int a = ab / W - d.ne, b = ab % W - d.ne;
+    DNS[j] = DN[(k * d.lx + offset_bin(ux, a, nsb)) * d.ly + offset_bin(uy, b, nsb)];
+  }
+}
+
 __global__ void k_lut_tm3(Dev d, const float* __restrict__ TM, float3* __restrict__ TM3) {
   const int64_t cc = (int64_t)d.cm * d.cm;
   const int64_t n = cc * d.nm * kWinPad;

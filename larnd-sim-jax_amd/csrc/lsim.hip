@@ -63,6 +63,7 @@ struct lsim_ctx {
   float* TNP = nullptr;
   float2* DM = nullptr;
   float2* DN = nullptr;
+  float2* DNS = nullptr;  // DN regrouped per (tick, sub-bin x, sub-bin y): one segment's neighbour entries contiguous
   double* CTM = nullptr;
   double* CTN = nullptr;
 };
@@ -330,11 +331,12 @@ void free_tables(lsim_ctx* c) {
   (void)hipFree(c->TNP);
   (void)hipFree(c->DM);
   (void)hipFree(c->DN);
+  (void)hipFree(c->DNS);
   (void)hipFree(c->CTM);
   (void)hipFree(c->CTN);
   c->TM = c->TN = c->TNP = nullptr;
   c->TM3 = nullptr;
-  c->DM = c->DN = nullptr;
+  c->DM = c->DN = c->DNS = nullptr;
   c->CTM = c->CTN = nullptr;
   c->has_lut = false;
 }
@@ -437,6 +439,7 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
   c->ly = ny;
   c->nt = nt;
   const int64_t ms = (int64_t)k.n_templates * c->cm * c->cm, ns = (int64_t)nx * ny;
+  const int64_t nds = (int64_t)nsb * nsb * (2 * n + 1) * (2 * n + 1);  // DNS entries per tick
   const int L = k.signal_length;
   Dev d;
   int rc = make_dev(c, nullptr, 1, d);
@@ -448,6 +451,7 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
         {(void**)&c->TM3, ms * kWinPad * sizeof(float3)}, {(void**)&c->TM, ms * L * sizeof(float)},
         {(void**)&c->TN, ns * L * sizeof(float)},         {(void**)&c->TNP, ns * kWinPad * sizeof(float)},
         {(void**)&c->DM, ms * nt * sizeof(float2)},       {(void**)&c->DN, ns * nt * sizeof(float2)},
+        {(void**)&c->DNS, nds * nt * sizeof(float2)},
         {(void**)&c->CTM, ms * (L + 1) * sizeof(double)}, {(void**)&c->CTN, ns * (L + 1) * sizeof(double)}};
     for (const auto& q : req) {
       const hipError_t e = hipMalloc(q.first, q.second);
@@ -461,6 +465,7 @@ int lsim_lut_set(lsim_ctx* c, const float* raw, int32_t nx, int32_t ny, int32_t 
   hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ms), dim3(256), 0, st, d, raw, 1, c->TM, (float*)nullptr, c->DM, c->CTM);
   hipLaunchKernelGGL(k_lut_tables, dim3((unsigned)ns), dim3(256), 0, st, d, raw, 0, c->TN, c->TNP, c->DN, c->CTN);
   hipLaunchKernelGGL(k_lut_tm3, dim3(1024), dim3(256), 0, st, d, c->TM, c->TM3);
+  hipLaunchKernelGGL(k_lut_dns, dim3(4096), dim3(256), 0, st, d, c->DN, c->DNS);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     free_tables(c);  // no half-built tables stay behind (and none leak on the next call)
@@ -521,11 +526,11 @@ int lsim_forward_prepare(lsim_ctx* c, const lsim_params* p, const lsim_segments*
     uint32_t *keys, *vals;
     sorted_buffers(w, d, keys, vals);
     if (d.nb == 5) {
-      hipLaunchKernelGGL(k_records<5>, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, c->DN, c->DM, w.rec, w.prs, w.dtot);
+      hipLaunchKernelGGL(k_records<5>, dim3(g), dim3(kRecThreads), 0, st, d, in, keys, vals, n, c->DNS, c->DM, w.rec, w.prs, w.dtot);
     } else if (d.nb == 3) {
-      hipLaunchKernelGGL(k_records<3>, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, c->DN, c->DM, w.rec, w.prs, w.dtot);
+      hipLaunchKernelGGL(k_records<3>, dim3(g), dim3(kRecThreads), 0, st, d, in, keys, vals, n, c->DNS, c->DM, w.rec, w.prs, w.dtot);
     } else {
-      hipLaunchKernelGGL(k_records<1>, dim3(g), dim3(256), 0, st, d, in, keys, vals, n, c->DN, c->DM, w.rec, w.prs, w.dtot);
+      hipLaunchKernelGGL(k_records<1>, dim3(g), dim3(kRecThreads), 0, st, d, in, keys, vals, n, c->DNS, c->DM, w.rec, w.prs, w.dtot);
     }
     hipLaunchKernelGGL(k_cells, dim3(g), dim3(256), 0, st, d, keys, n, w.rng, w.flag);
     HIP_TRY((exclusive_scan<int, int>(w.flag, w.ridx, n, w.scan_tmp, w.n_rows, st)));
