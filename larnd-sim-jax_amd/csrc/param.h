@@ -125,6 +125,34 @@ __device__ __forceinline__ double param_tail(double gj, double loc, double scale
   return (Gn - G + exp(fmin(0.0, (loc - dt / 2) / scale)) / T) / dt;
 }
 
+// Both tails of a pair's current (decays sb and sc) at sample k: param_tail of each, with the
+// Gaussian part 1/2 erf((g - mu) / (sqrt2 sigma)) of the two CDFs evaluated once (the same operations
+// as emg_cdf_half, so the values are the same).
+__device__ __forceinline__ void param_tails(double gj, double loc, double sb, double sc, double sigma, double dt, int T,
+                                            bool diffusion, double& tb, double& tc) {
+  if (!diffusion) {
+    tb = param_tail(gj, loc, sb, sigma, dt, T, false);
+    tc = param_tail(gj, loc, sc, sigma, dt, T, false);
+    return;
+  }
+  const double half = 0.5 * erf((gj - loc) / (M_SQRT2 * sigma));
+  double F[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double lambd = 1.0 / (k == 0 ? sb : sc);
+    const double coeff = lambd / 2;
+    const double ex = coeff * (2 * loc + lambd * sigma * sigma - 2 * gj);
+    const double pdf = coeff * exp(ex) * erfc((loc + lambd * sigma * sigma - gj) / (M_SQRT2 * sigma));
+    F[k] = half - pdf / lambd;
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double Fn = __shfl_down(F[k], 1, kWave);
+    const double F1 = __shfl(F[k], 1, kWave), FT = __shfl(F[k], T - 1, kWave);
+    (k == 0 ? tb : tc) = (F[k] - Fn) / (F1 - FT) / dt;
+  }
+}
+
 // One workgroup per row (sorted unique pixel id upix[r]): the pairs order[offs[r] .. offs[r+1]) in
 // pair order, pair i going to wave i % kParamWaves; lane k adds sample k of the pair's current
 // times its charge at waveform tick t0_tick - T + k (detsim_jax.py:209-228: the ticks outside
@@ -167,8 +195,9 @@ __global__ __launch_bounds__(kParamWaves* kWave) void k_param_rows(Dev d, ParamE
       const int t0_tick = (int)(t0 / d.ts + 0.5);
       const double st0 = (t0 - t0_tick * d.ts) + ts0;
       const double sig = e.sig[el];
-      const double cur = a * param_tail(gj, -st0, b, sig, dt, T, diffusion != 0) +
-                         (1 - a) * param_tail(gj, -st0, c, sig, dt, T, diffusion != 0);
+      double tlb, tlc;
+      param_tails(gj, -st0, b, c, sig, dt, T, diffusion != 0, tlb, tlc);
+      const double cur = a * tlb + (1 - a) * tlc;
       const int o = t0_tick - T + kk;
       if (lane < T && o >= 0 && o < nout) acc[wid][o] += cur * e.ne[el];
     }
