@@ -440,6 +440,8 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
                                                             const AdjRec* __restrict__ adj, const int2* __restrict__ rng,
                                                             const int* __restrict__ row_cell,
                                                             const int* __restrict__ n_rows_dev, int64_t max_rows,
+                                                            const int* __restrict__ band,
+                                                            const int* __restrict__ order,
                                                             const Pieces* __restrict__ pieces,
                                                             double* __restrict__ partial) {
   constexpr int NT = WAVES * kWave;
@@ -454,14 +456,13 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
   const int t = threadIdx.x;
   const int P0 = *n_rows_dev;
   const int P = P0 < max_rows ? P0 : (int)max_rows;
-  if ((int64_t)blockIdx.x >= P) {
-    if (blockIdx.x < max_rows && t < kNGrad) partial[(int64_t)blockIdx.x * kNGrad + t] = 0.0;
-    return;
-  }
-  // rows in XCD bands, as in the forward
+  // the rows in the forward's XCD bands (equal estimated work per XCD, tile-major order within):
+  // workgroup i runs on XCD i % kXcd and takes slot i / kXcd of that XCD's band; the partials of
+  // rows no workgroup takes (past P) were zeroed by the host
   const int xcd = (int)(blockIdx.x % kXcd);
-  const int q8 = P / kXcd, rem = P % kXcd;
-  const int r = xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
+  const int slot = band[xcd] + (int)(blockIdx.x / kXcd);
+  if (slot >= band[xcd + 1] || slot >= P) return;
+  const int r = order[slot];
   const Pieces* pr = pieces + r;
   const int np = pr->n;
   {
@@ -638,6 +639,8 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj_dense(Dev d, Tables 
                                                                   const int2* __restrict__ rng,
                                                                   const int* __restrict__ row_cell,
                                                                   const int* __restrict__ n_rows_dev, int64_t max_rows,
+                                                                  const int* __restrict__ band,
+                                                                  const int* __restrict__ order,
                                                                   const Pieces* __restrict__ pieces,
                                                                   const float* __restrict__ gw,
                                                                   double* __restrict__ partial) {
@@ -656,16 +659,11 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj_dense(Dev d, Tables 
   const int P0 = *n_rows_dev;
   const int P = P0 < max_rows ? P0 : (int)max_rows;
   double* out_near = partial + max_rows * kNGrad;  // the second half of the partials: zero here
-  if ((int64_t)blockIdx.x >= P) {
-    if (blockIdx.x < max_rows && t < kNGrad) {
-      partial[(int64_t)blockIdx.x * kNGrad + t] = 0.0;
-      out_near[(int64_t)blockIdx.x * kNGrad + t] = 0.0;
-    }
-    return;
-  }
+  // the rows in the forward's XCD bands, as in k_rows_adj (partials of the rows past P pre-zeroed)
   const int xcd = (int)(blockIdx.x % kXcd);
-  const int q8 = P / kXcd, rem = P % kXcd;
-  const int r = xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
+  const int slot = band[xcd] + (int)(blockIdx.x / kXcd);
+  if (slot >= band[xcd + 1] || slot >= P) return;
+  const int r = order[slot];
   const int nout = d.nout, L = d.L;
   // G over the padded tick range: window ticks 1 .. nout - 1, zero elsewhere
   if (t == 0) s_any = 0;

@@ -831,21 +831,28 @@ static int backward_impl(lsim_ctx* c, const lsim_params* p, const lsim_segments*
   uint32_t *keys, *vals;
   sorted_buffers(w, d, keys, vals);
   if (max_rows > 0) {
+    // the rows' processing order and XCD bands, rebuilt as the forward builds them (the schedule
+    // settings may have changed since; the result does not depend on them: the partials are
+    // reduced by row); the rows no workgroup takes keep zero partials
+    const int rc2 = launch_row_bands(d, w, max_rows, st);
+    if (rc2) return rc2;
+    fill<double>(partial, 2 * max_rows * kNGrad, 0.0, st);
+    const int* order = row_order(w, d);
     hipLaunchKernelGGL(k_fee_adjoint, dim3((unsigned)((max_rows + kFeeRows - 1) / kFeeRows)), dim3(kFeeRows * kWave), 0, st,
                        d, (const FeeTape*)fwd->tape,
                        d_adc, d_z, fwd->n_rows, max_rows, pieces, dz);
     if (n > 0)
       hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, in, keys, vals, n, adj);
-    const dim3 grid((unsigned)max_rows), block(kAdjWaves * kWave);
+    const dim3 grid((unsigned)(kXcd * band_slots(max_rows))), block(kAdjWaves * kWave);
     double* pnear = partial + max_rows * kNGrad;
 #define LSIM_ADJ_LAUNCH(NBV)                                                                                        \
   hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,       \
-                     w.row_cell, w.n_rows, max_rows, pieces, partial);                                              \
+                     w.row_cell, w.n_rows, max_rows, w.band, order, pieces, partial);                               \
   hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,        \
-                     w.row_cell, w.n_rows, max_rows, pieces, pnear)
+                     w.row_cell, w.n_rows, max_rows, w.band, order, pieces, pnear)
 #define LSIM_ADJ_DENSE(NBV)                                                                                       \
   hipLaunchKernelGGL((k_rows_adj_dense<NBV, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,       \
-                     w.row_cell, w.n_rows, max_rows, pieces, d_wfs, partial)
+                     w.row_cell, w.n_rows, max_rows, w.band, order, pieces, d_wfs, partial)
     if (d_wfs) {
       if (d.nb == 5) {
         LSIM_ADJ_DENSE(5);
