@@ -408,6 +408,20 @@ __device__ __forceinline__ bool xbin_adjoint(const Dev& d, const Tables& tb, con
   return true;
 }
 
+// The row of this workgroup in the adjoint row kernels (-1: none): slot blockIdx / kXcd of XCD
+// blockIdx % kXcd's band of the processing order, or with no bands (band == nullptr) the XCD's
+// equal share of the rows in row order.
+__device__ __forceinline__ int adj_row(const int* __restrict__ band, const int* __restrict__ order, int P) {
+  const int xcd = (int)(blockIdx.x % kXcd);
+  if (band) {
+    const int slot = band[xcd] + (int)(blockIdx.x / kXcd);
+    return (slot >= band[xcd + 1] || slot >= P) ? -1 : order[slot];
+  }
+  if ((int64_t)blockIdx.x >= P) return -1;
+  const int q8 = P / kXcd, rem = P % kXcd;
+  return xcd * q8 + (xcd < rem ? xcd : rem) + (int)(blockIdx.x / kXcd);
+}
+
 // Chain rule to the 7 parameters from the pair's d loss / d (n_e, f, sigma_T, lambda).
 __device__ __forceinline__ void add_chain(const AdjRec* __restrict__ Ap, double gN, double gF, double gS, double gL,
                                           bool near, double* g) {
@@ -456,13 +470,12 @@ __global__ __launch_bounds__(WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4
   const int t = threadIdx.x;
   const int P0 = *n_rows_dev;
   const int P = P0 < max_rows ? P0 : (int)max_rows;
-  // the rows in the forward's XCD bands (equal estimated work per XCD, tile-major order within):
-  // workgroup i runs on XCD i % kXcd and takes slot i / kXcd of that XCD's band; the partials of
-  // rows no workgroup takes (past P) were zeroed by the host
-  const int xcd = (int)(blockIdx.x % kXcd);
-  const int slot = band[xcd] + (int)(blockIdx.x / kXcd);
-  if (slot >= band[xcd + 1] || slot >= P) return;
-  const int r = order[slot];
+  // workgroup i runs on XCD i % kXcd.  With bands (large launches): the forward's XCD bands of
+  // equal estimated work in tile-major order, slot i / kXcd of XCD i % kXcd's band; without (small
+  // launches, where rebuilding them costs more than it saves): equal row counts per XCD in row
+  // order.  The partials of rows no workgroup takes (past P) were zeroed by the host.
+  const int r = adj_row(band, order, P);
+  if (r < 0) return;
   const Pieces* pr = pieces + r;
   const int np = pr->n;
   {
@@ -659,11 +672,8 @@ __global__ __launch_bounds__(WAVES * kWave) void k_rows_adj_dense(Dev d, Tables 
   const int P0 = *n_rows_dev;
   const int P = P0 < max_rows ? P0 : (int)max_rows;
   double* out_near = partial + max_rows * kNGrad;  // the second half of the partials: zero here
-  // the rows in the forward's XCD bands, as in k_rows_adj (partials of the rows past P pre-zeroed)
-  const int xcd = (int)(blockIdx.x % kXcd);
-  const int slot = band[xcd] + (int)(blockIdx.x / kXcd);
-  if (slot >= band[xcd + 1] || slot >= P) return;
-  const int r = order[slot];
+  const int r = adj_row(band, order, P);  // as in k_rows_adj (partials of the rows past P pre-zeroed)
+  if (r < 0) return;
   const int nout = d.nout, L = d.L;
   // G over the padded tick range: window ticks 1 .. nout - 1, zero elsewhere
   if (t == 0) s_any = 0;

@@ -802,6 +802,10 @@ int lsim_backward_query(lsim_ctx* c, int64_t n_seg, int64_t max_rows, size_t* by
   return LSIM_OK;
 }
 
+// Row count from which the adjoint's row kernels take the forward's work-balanced bands (rebuilt
+// per backward, about 0.1 ms); below it, equal row counts per XCD.
+constexpr int64_t kAdjBandRows = 32768;
+
 static int backward_impl(lsim_ctx* c, const lsim_params* p, const lsim_segments* seg, int32_t n_events, void* ws,
                          size_t ws_bytes, int64_t max_rows, const lsim_fwd_out* fwd, const float* d_adc, const float* d_z,
                          const float* d_wfs, void* bwd_ws, size_t bwd_ws_bytes, double* d_params, void* stream) {
@@ -831,28 +835,32 @@ static int backward_impl(lsim_ctx* c, const lsim_params* p, const lsim_segments*
   uint32_t *keys, *vals;
   sorted_buffers(w, d, keys, vals);
   if (max_rows > 0) {
-    // the rows' processing order and XCD bands, rebuilt as the forward builds them (the schedule
-    // settings may have changed since; the result does not depend on them: the partials are
-    // reduced by row); the rows no workgroup takes keep zero partials
-    const int rc2 = launch_row_bands(d, w, max_rows, st);
-    if (rc2) return rc2;
+    // large launches: the rows' processing order and XCD bands, rebuilt as the forward builds them
+    // (the schedule settings may have changed since; the result does not depend on them: the
+    // partials are reduced by row); the rows no workgroup takes keep zero partials
+    const bool banded = max_rows >= kAdjBandRows;
+    if (banded) {
+      const int rc2 = launch_row_bands(d, w, max_rows, st);
+      if (rc2) return rc2;
+    }
     fill<double>(partial, 2 * max_rows * kNGrad, 0.0, st);
-    const int* order = row_order(w, d);
+    const int* band = banded ? w.band : nullptr;
+    const int* order = banded ? row_order(w, d) : nullptr;
     hipLaunchKernelGGL(k_fee_adjoint, dim3((unsigned)((max_rows + kFeeRows - 1) / kFeeRows)), dim3(kFeeRows * kWave), 0, st,
                        d, (const FeeTape*)fwd->tape,
                        d_adc, d_z, fwd->n_rows, max_rows, pieces, dz);
     if (n > 0)
       hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, in, keys, vals, n, adj);
-    const dim3 grid((unsigned)(kXcd * band_slots(max_rows))), block(kAdjWaves * kWave);
+    const dim3 grid((unsigned)(banded ? kXcd * band_slots(max_rows) : max_rows)), block(kAdjWaves * kWave);
     double* pnear = partial + max_rows * kNGrad;
 #define LSIM_ADJ_LAUNCH(NBV)                                                                                        \
   hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, false>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,       \
-                     w.row_cell, w.n_rows, max_rows, w.band, order, pieces, partial);                               \
+                     w.row_cell, w.n_rows, max_rows, band, order, pieces, partial);                               \
   hipLaunchKernelGGL((k_rows_adj<NBV, kAdjWaves, true>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,        \
-                     w.row_cell, w.n_rows, max_rows, w.band, order, pieces, pnear)
+                     w.row_cell, w.n_rows, max_rows, band, order, pieces, pnear)
 #define LSIM_ADJ_DENSE(NBV)                                                                                       \
   hipLaunchKernelGGL((k_rows_adj_dense<NBV, kAdjWaves>), grid, block, 0, st, d, tb, w.rec, w.prs, adj, w.rng,       \
-                     w.row_cell, w.n_rows, max_rows, w.band, order, pieces, d_wfs, partial)
+                     w.row_cell, w.n_rows, max_rows, band, order, pieces, d_wfs, partial)
     if (d_wfs) {
       if (d.nb == 5) {
         LSIM_ADJ_DENSE(5);
