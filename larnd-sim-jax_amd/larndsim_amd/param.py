@@ -83,11 +83,14 @@ def waveforms(params, e):
 
 def _waveforms(params, sim, e):
     pids = e["pair_pids"].reshape(-1)
-    U, inv = torch.unique(pids, sorted=True, return_inverse=True)
+    # one stable sort of the pair pixel ids gives the rows (the distinct ids, ascending), each row's
+    # pairs in pair order and the row offsets
+    spids, order = torch.sort(pids, stable=True)
+    U, counts = torch.unique_consecutive(spids, return_counts=True)
     P = U.numel()
-    order = torch.argsort(inv, stable=True).to(torch.int32)
+    order = order.to(torch.int32)
     offs = torch.zeros(P + 1, dtype=torch.int32, device=pids.device)
-    offs[1:] = torch.cumsum(torch.bincount(inv, minlength=P), 0).to(torch.int32)
+    offs[1:] = torch.cumsum(counts, 0).to(torch.int32)
     wfs = torch.empty((P, sim.nout), dtype=torch.float32, device=pids.device)
     vp = lambda a: C.c_void_p(a.data_ptr())  # noqa: E731
     _lib.check(sim.lib.lsim_param_rows(sim._ctx, C.byref(params_struct(params)), vp(e["x"]), vp(e["y"]), vp(e["t0"]),
